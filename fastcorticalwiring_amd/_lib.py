@@ -1,0 +1,174 @@
+"""ctypes binding of libfcw.so (include/fcw.h).
+
+The library is built in-tree (``fastcorticalwiring_amd/libfcw.so``) by
+``fastcorticalwiring_amd/build.py``; importing this module never builds or
+falls back to anything else: if the library is missing, ``lib()`` raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfcw.so")
+
+FCW_FP64 = 0
+FCW_FP32_IR = 1
+
+(S_FLAG, S_NEG_FRAC, S_VIOL_FRAC, S_DIST_BOUNDARY, S_MSD_U, S_MSD_W, S_N_EVAL, S_N_SAMPLES,
+ S_OVERFLOW, NSCALARS) = range(10)
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_double),
+        ("t_coef", ctypes.c_double),
+        ("poisson_shift", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("block", ctypes.c_int),
+        ("leaf_size", ctypes.c_int),
+        ("host_threads", ctypes.c_int),
+        ("device", ctypes.c_int),
+        ("ir_steps", ctypes.c_int),
+        ("reserved", ctypes.c_int * 6),
+    ]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("nnodes", ctypes.c_int),
+        ("nlevels", ctypes.c_int),
+        ("max_front", ctypes.c_int),
+        ("block", ctypes.c_int),
+        ("factor_nnz", ctypes.c_int64),
+        ("panel_elems", ctypes.c_int64),
+        ("factor_flops", ctypes.c_double),
+        ("solve_flops_per_rhs", ctypes.c_double),
+        ("t", ctypes.c_double),
+        ("mean_edge", ctypes.c_double),
+        ("setup_seconds", ctypes.c_double * 4),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "setup_seconds"}
+        d["setup_seconds"] = list(self.setup_seconds)
+        return d
+
+
+class MetricsOut(ctypes.Structure):
+    _fields_ = [
+        ("scalars", ctypes.POINTER(ctypes.c_double)),
+        ("radii", ctypes.POINTER(ctypes.c_double)),
+        ("perimeters", ctypes.POINTER(ctypes.c_double)),
+        ("extra_r", ctypes.POINTER(ctypes.c_double)),
+        ("extra_a", ctypes.POINTER(ctypes.c_double)),
+        ("sample_r", ctypes.POINTER(ctypes.c_float)),
+        ("sample_a", ctypes.POINTER(ctypes.c_float)),
+    ]
+
+
+# Every symbol include/fcw.h declares.
+EXPORTED_SYMBOLS = (
+    "fcw_default_options", "fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block",
+    "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
+    "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
+)
+
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load libfcw.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libfcw.so not found at {LIB_PATH}; build it with "
+            "`python -m fastcorticalwiring_amd.build` (hipcc, gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    L.fcw_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.fcw_default_options.restype = None
+    L.fcw_create.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(Options), ctypes.POINTER(_vp)]
+    L.fcw_destroy.argtypes = [_vp]
+    L.fcw_get_info.argtypes = [_vp, ctypes.POINTER(Info)]
+    L.fcw_distance_block.argtypes = [_vp, _ip, ctypes.c_int, _dp]
+    L.fcw_set_metrics.argtypes = [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int]
+    L.fcw_wiring_sources.argtypes = [_vp, _ip, ctypes.c_int, ctypes.POINTER(MetricsOut)]
+    L.fcw_geometry_fields.argtypes = [_vp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.POINTER(MetricsOut)]
+    L.fcw_stage_times.argtypes = [_vp, _dp, ctypes.c_int]
+    L.fcw_set_profiling.argtypes = [_vp, ctypes.c_int]
+    L.fcw_kernel_times.argtypes = [_vp, _dp, ctypes.c_int]
+    L.fcw_build_info.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.fcw_last_error.argtypes = []
+    L.fcw_last_error.restype = ctypes.c_char_p
+    L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
+                 "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
+                 "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve"):
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+class FcwError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().fcw_last_error().decode("utf-8", "replace")
+        raise FcwError(msg or f"libfcw error {rc}")
+
+
+def build_info() -> str:
+    buf = ctypes.create_string_buffer(512)
+    lib().fcw_build_info(buf, 512)
+    return buf.value.decode()
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def host_factor_solve(V, F, l_scale, m_scale, B, leaf_size=64):
+    """CPU-only check of the analysis + FP64 factorisation: X = (l*L + m*M)^{-1} B (B is n x k)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B[:, None]
+    X = np.empty_like(B)
+    stats = np.zeros(4)
+    check(lib().fcw_host_factor_solve(dptr(V), V.shape[0], iptr(F), F.shape[0], float(l_scale), float(m_scale),
+                                      int(leaf_size), dptr(B), B.shape[1], dptr(X), dptr(stats)))
+    return X, dict(nnodes=int(stats[0]), nlevels=int(stats[1]), factor_nnz=int(stats[2]),
+                   panel_elems=int(stats[3]))

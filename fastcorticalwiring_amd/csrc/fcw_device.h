@@ -1,0 +1,100 @@
+// Device kernel launchers (kernels.hip) used by the C-ABI runtime (fcw_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fcw {
+
+struct LevelTile {
+    int node;
+    int m0;
+};
+
+// Tiling of the level panel products (rows x columns x depth per workgroup).
+constexpr int kTileM = 64;
+constexpr int kTileN = 64;
+constexpr int kTileK = 16;
+constexpr int kColPad = 64;  // RHS block widths are padded to this many columns
+
+// Metric kernel limits.
+constexpr int kMaxScales = 32;
+constexpr int kMaxExtras = 256;
+constexpr int kMaxBins = 2048;
+constexpr int kMetricThreads = 1024;
+
+// Per-source scalar outputs, in this order (double each).
+enum ScalarSlot {
+    kSlotFlag = 0,
+    kSlotNeg,
+    kSlotViol,
+    kSlotDistB,
+    kSlotMsdU,
+    kSlotMsdW,
+    kSlotNEval,
+    kSlotNSamples,
+    kSlotOverflow,
+    kSlotCount
+};
+
+struct MeshDev {
+    int n, m;
+    const double* V;        // n x 3, positions
+    const int* faces;       // m x 3, positions
+    const double* area;     // m (cross-product area)
+    const double* gb;       // m x 9 (grad B0, grad B1, grad B2)
+    const double* larea;    // m (Gram area, metric kernels)
+    const double* g11;
+    const double* g12;
+    const double* g22;
+    const int* vf_ptr;      // n + 1
+    const int* vf;          // face*3 + corner
+};
+
+struct MetricsDev {
+    int S, q, nb;
+    const double* targets;  // S (ascending)
+    const double* fracs;    // q
+    const int* boundary;    // nb, positions
+    const double* vw;       // n, vertex areas (positions)
+    double eps, health_h, width, cap;  // cap < 0: no cap
+    double rtol;
+    int max_iter;
+    int band_cap;           // entries per scale per source in the band scratch
+    int* band;              // k x S x band_cap
+};
+
+struct MetricsOut {
+    double* scal;      // k x kSlotCount
+    double* radii;     // k x S
+    double* perim;     // k x S
+    double* extra_r;   // k x (S-1)q
+    double* extra_a;   // k x (S-1)q
+    float* samp_r;     // k x (S + (S-1)q)
+    float* samp_a;     // k x (S + (S-1)q)
+};
+
+void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gb,
+                          double* larea, double* g11, double* g12, double* g22);
+void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, const int* contrib, const double* area,
+                     const double* gb, const int* vf_ptr, const int* vf, double* lval, double* mass);
+void launch_fill(hipStream_t st, double* p, int64_t count, double value);
+void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);
+void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
+                         const int* node_s, const int* node_b, const int64_t* node_boff, const int64_t* node_uoff,
+                         const int* child_ptr, const int* child_list, const int* cmap, double* X, double* U);
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
+                     const double* panel, const double* X, double* Y, double* U, bool accumulate);
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z);
+void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out);
+void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
+                            double* Dt);
+void launch_gather_rows(hipStream_t st, int n, int k, const double* Dt, const int* iperm, double* out);
+void launch_metrics(hipStream_t st, int k, const MeshDev& mesh, const MetricsDev& md, const double* Dt,
+                    const int* src_pos, MetricsOut out, bool health_screen);
+
+}  // namespace fcw

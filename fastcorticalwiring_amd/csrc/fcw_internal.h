@@ -1,0 +1,65 @@
+// Internal structures shared by the host analysis (symbolic.cpp, numeric.cpp)
+// and the device runtime (fcw_api.hip). Not part of the public C-ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace fcw {
+
+// Separator-tree supernodal structure of the cotan stiffness pattern.
+//
+// Vertices are renumbered by a nested-dissection ordering ("positions").
+// Every tree node owns a contiguous position range [start, start + s) (its
+// separator, or a whole leaf subdomain) and a sorted list of b boundary
+// positions > start + s (the ancestor vertices adjacent to its subtree).
+// Its factor panel is dense (s + b) x s, column-major.
+struct Symbolic {
+    int n = 0;                    // vertices
+    int m = 0;                    // faces
+    std::vector<int> perm;        // perm[p] = input vertex at position p
+    std::vector<int> iperm;       // iperm[v] = position of input vertex v
+
+    int nnodes = 0;
+    std::vector<int> node_start, node_s, node_b, node_parent, node_level;
+    std::vector<int64_t> node_boff;       // offset into bidx / cmap
+    std::vector<int> bidx;                // boundary positions, per node, ascending
+    std::vector<int> cmap;                // per node: its boundary rows -> parent front rows
+    std::vector<int64_t> node_poff;       // panel offset (elements)
+    std::vector<int64_t> node_uoff;       // update-buffer row offset
+    std::vector<int> child_ptr, child_list;
+    int nlevels = 0;
+    std::vector<int> level_ptr, level_nodes;  // nodes grouped by level (leaves = level 0)
+    int64_t panel_elems = 0;
+    int64_t urows = 0;
+
+    // Full symmetric CSR of the operator pattern in positions (sorted columns, with diagonal).
+    std::vector<int> rowptr, colind, diag;
+    // Per CSR entry: contributing (face*9 + a*3 + b) with faces in position numbering.
+    std::vector<int> contrib_ptr, contrib;
+    // Faces in position numbering, reordered for locality.
+    std::vector<int> faces;               // m x 3
+    std::vector<int> face_perm;           // face_perm[f_new] = input face index
+    // Vertex -> incident (face*3 + corner), position numbering.
+    std::vector<int> vf_ptr, vf;
+
+    // Work counts.
+    double factor_flops = 0.0;       // one numeric factorisation (incl. inverse panels)
+    double solve_flops_per_rhs = 0.0;  // one forward + one backward sweep, one RHS column
+    int64_t factor_nnz = 0;          // sum over nodes of s(s+1)/2 + s*b (Cholesky factor entries)
+    int max_front = 0;
+};
+
+int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err);
+
+// Multifrontal FP64 Cholesky of A = diag_scale*Mdiag + off_scale*L (pattern of S, values in CSR
+// order `lval`, lumped mass `mdiag` in positions). Writes the partitioned-inverse panels
+// P_S = [L11^{-1}; L21 L11^{-1}] into `panel` (S.panel_elems doubles).
+int factor(const Symbolic& S, const double* lval, const double* mdiag, double l_scale, double m_scale,
+           double* panel, std::string& err, int nthreads);
+
+// CPU application of the panels: solves A X = B in place for k columns (row-major n x k, positions).
+void panel_solve_host(const Symbolic& S, const double* panel, double* X, int k);
+
+}  // namespace fcw

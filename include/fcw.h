@@ -1,0 +1,141 @@
+/*
+ * fcw.h - C ABI of libfcw.so, the MI355X (gfx950) heat-method geodesic and
+ * wiring-metric engine. Plain C types only; the caller owns every host buffer,
+ * the library owns device memory per context. Every call returns 0 on success
+ * and a nonzero status on failure, with the message in fcw_last_error()
+ * (thread-local). One context = one HIP stream; calls on one context must be
+ * serialised by the caller, contexts are independent.
+ *
+ * Vertex and face arrays describe the cortical submesh, as the reference hands
+ * it to its distance engine (core_analysis.py:646-667).
+ */
+#ifndef FCW_H
+#define FCW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fcw_ctx fcw_ctx;
+
+/* Arithmetic of the two triangular sweeps per system. */
+enum {
+    FCW_FP64 = 0,    /* plain FP64 panels and RHS blocks */
+    FCW_FP32_IR = 1  /* FP32 panel products, FP64 residual refinement */
+};
+
+/* Scalar outputs per source, in this order (fcw_metrics_out.scalars is k x FCW_NSCALARS). */
+enum {
+    FCW_S_FLAG = 0,      /* 1 if the field failed the health screen (lean_geometry.py:343-357) */
+    FCW_S_NEG_FRAC,      /* fraction of vertices with d < -h */
+    FCW_S_VIOL_FRAC,     /* fraction with d < 0.9 |x - x_src| - h */
+    FCW_S_DIST_BOUNDARY, /* min finite d over boundary vertices; +inf on a closed mesh */
+    FCW_S_MSD_U,         /* mean d over vertices with d > eps */
+    FCW_S_MSD_W,         /* vertex-area weighted mean over the same vertices */
+    FCW_S_N_EVAL,        /* band evaluations used by the radius/perimeter/sample solves */
+    FCW_S_N_SAMPLES,     /* cleaned (radius, area) pairs written for this source */
+    FCW_S_OVERFLOW,      /* 1 if a band list overflowed and the full-face fallback ran */
+    FCW_NSCALARS
+};
+
+typedef struct {
+    double t;              /* diffusion time; <= 0 selects t_coef * mean_edge^2 */
+    double t_coef;         /* default 1.0 (fastcw_lean.py:481, 485-489) */
+    double poisson_shift;  /* default 1e-10: Poisson matrix L + shift*M (fastcw_lean.py:500) */
+    int precision;         /* FCW_FP64 or FCW_FP32_IR */
+    int block;             /* maximum sources per RHS block (padded to a multiple of 64); default 256 */
+    int leaf_size;         /* nested-dissection leaf size; default 64 */
+    int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
+    int device;            /* HIP device ordinal */
+    int ir_steps;          /* refinement sweeps per system in FCW_FP32_IR mode; default 2 */
+    int reserved[6];
+} fcw_options;
+
+typedef struct {
+    int n, m, nnodes, nlevels, max_front, block;
+    int64_t factor_nnz;         /* sum over supernodes of s(s+1)/2 + s*b */
+    int64_t panel_elems;        /* stored panel entries per system */
+    double factor_flops;        /* per numeric factorisation */
+    double solve_flops_per_rhs; /* executed panel-product flops, one system, forward + backward */
+    double t, mean_edge;
+    double setup_seconds[4];    /* analyse, assemble, factor (both systems), upload */
+    int64_t device_bytes;       /* device memory held by the context */
+} fcw_info;
+
+typedef struct {
+    double* scalars;     /* k x FCW_NSCALARS */
+    double* radii;       /* k x S */
+    double* perimeters;  /* k x S */
+    double* extra_r;     /* k x (S-1)q */
+    double* extra_a;     /* k x (S-1)q */
+    float* sample_r;     /* k x (S + (S-1)q), first scalars[FCW_S_N_SAMPLES] entries valid */
+    float* sample_a;     /* k x (S + (S-1)q) */
+} fcw_metrics_out;
+
+void fcw_default_options(fcw_options* opt);
+
+/* Build operators, factor M + tL and L + shift*M, upload everything.
+ * Replaces pp3d.MeshHeatMethodDistanceSolver(V, F, t_coef=..., use_robust=...)
+ * (distance_engines.py:119-125) and ConsistentBatchHeatEngine.__init__
+ * (fastcw_lean.py:481-502). Zero-area faces are rejected as the latter does
+ * (fastcw_lean.py:383-384). */
+int fcw_create(const double* V, int n, const int* F, int m, const fcw_options* opt, fcw_ctx** out);
+int fcw_destroy(fcw_ctx* ctx);
+int fcw_get_info(const fcw_ctx* ctx, fcw_info* out);
+
+/* Distances from k sources, row per source: out is k x n float64 (input vertex order), shifted to
+ * 0 at each source. Replaces compute_distance (distance_engines.py:131-133) and
+ * ConsistentBatchHeatEngine.compute_distance_batch (fastcw_lean.py:504-531). k <= block. */
+int fcw_distance_block(fcw_ctx* ctx, const int* src, int k, double* out);
+
+/* Per-source metric configuration (core_analysis.py:1513-1514, 1536, 1588, 705):
+ * targets = sorted scale * total area (S <= 32), fracs = extra-sample fractions (q), boundary =
+ * submesh boundary vertices, vertex_w = vertex areas (n), eps = MSD threshold (1e-6), health_h =
+ * median face max-edge, bin_width = lean bucket width, cap_fraction < 0 disables the boundary cap,
+ * rtol / max_iter = radius solve tolerance (1e-10, 60). */
+int fcw_set_metrics(fcw_ctx* ctx, const double* targets, int S, const double* fracs, int q, const int* boundary,
+                    int nb, const double* vertex_w, double eps, double health_h, double bin_width,
+                    double cap_fraction, double rtol, int max_iter);
+
+/* The per-source loop body of compute_all_wiring_costs (core_analysis.py:1550-1609) for nsrc sources,
+ * processed in RHS blocks: distance -> health screen -> distance to boundary -> MSD -> radii,
+ * perimeters, supplementary samples -> cleaned samples. Output rows follow `src`. */
+int fcw_wiring_sources(fcw_ctx* ctx, const int* src, int nsrc, fcw_metrics_out* out);
+
+/* Metrics of caller-supplied fields D (k x n, input vertex order): the batched counterpart of
+ * lean_geometry.per_source_geometry (lean_geometry.py:302-340) when health_screen = 0, and of the
+ * whole per-source loop body on given fields (core_analysis.py:1558-1609) when health_screen = 1.
+ * src may be NULL (chord distances then use vertex 0). k <= block. */
+int fcw_geometry_fields(fcw_ctx* ctx, const double* D, const int* src, int k, int health_screen,
+                        fcw_metrics_out* out);
+
+/* Accumulated seconds: [0] heat_solve, [1] fused_div, [2] poisson_solve, [3] metrics, [4] wall of the
+ * last fcw_wiring_sources call (the stage keys of fastcw_lean.py:527-530 plus the metric stage).
+ * Stage times are filled only when profiling is on. Returns the number of values written. */
+int fcw_stage_times(fcw_ctx* ctx, double* out, int len);
+
+/* Per-kernel timing with HIP events around every launch on the context's stream (0 = off).
+ * fcw_kernel_times writes (total_ms, launches) for the forward and backward panel-product kernels
+ * and the metric kernel: out[0..5]. */
+int fcw_set_profiling(fcw_ctx* ctx, int on);
+int fcw_kernel_times(fcw_ctx* ctx, double* out, int len);
+
+/* Library self-description for the backend self-check; replaces the binary inspection of
+ * backend_check.verify_suitesparse (backend_check.py:32-73). */
+int fcw_build_info(char* buf, int len);
+const char* fcw_last_error(void);
+
+/* Host-only check of the analysis and factorisation (no GPU): solves A X = B for
+ * A = l_scale*L + m_scale*M assembled on the host from (V, F), with the CPU application of the
+ * partitioned-inverse panels. B/X are n x k row-major in input vertex order. stats[0..3] =
+ * nnodes, nlevels, factor_nnz, panel_elems. */
+int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale,
+                          int leaf_size, const double* B, int k, double* X, double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCW_H */
