@@ -33,7 +33,8 @@ class Options(ctypes.Structure):
         ("host_threads", ctypes.c_int),
         ("device", ctypes.c_int),
         ("ir_steps", ctypes.c_int),
-        ("reserved", ctypes.c_int * 6),
+        ("streams", ctypes.c_int),
+        ("reserved", ctypes.c_int * 5),
     ]
 
 

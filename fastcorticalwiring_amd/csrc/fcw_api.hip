@@ -1,6 +1,11 @@
 // C-ABI runtime of libfcw.so: context lifetime, device memory, and the
 // per-block pipeline  delta RHS -> heat sweeps -> fused divergence ->
 // Poisson sweeps -> shift/transpose -> metric kernel.
+//
+// A context owns the mesh, operators and both factor panel sets (shared), and
+// `streams` workspaces, each with its own HIP stream and RHS-block buffers, so
+// consecutive RHS blocks run concurrently: the narrow top-of-tree levels of one
+// block overlap with the wide leaf levels and metric kernel of another.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,15 +74,36 @@ struct EventPair {
     hipEvent_t a, b;
 };
 
+// Kernels timed per launch when profiling is on.
+enum TimedKernel { kFwdGemm = 0, kBwdGemm = 1, kMetrics = 2, kNumTimed = 3 };
+
+// Active-node plan of one block's heat forward sweep: only supernodes on the paths from the
+// block's sources to the root see a nonzero right-hand side.
+struct BlockPlan {
+    std::vector<int> tile_off;  // per level, into the call's tile array
+    std::vector<int> node_off;  // per level, into the call's node array
+    size_t flag_off = 0;        // into the call's flag array (nnodes bytes per block)
+};
+
 }  // namespace
+
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DBuf<double> X, Y, Z, U, Dt, band;
+    DBuf<int> src_pos;
+    std::vector<EventPair> ev_pool;
+    std::vector<std::pair<int, EventPair>> ev_pending;
+    size_t ev_used = 0;
+};
 
 struct fcw_ctx {
     fcw::Symbolic sym;
     fcw_options opt{};
     fcw_info info{};
     int n = 0, m = 0, ldk = 0;
-    hipStream_t stream = nullptr;
     int64_t dev_bytes = 0;
+    std::vector<std::unique_ptr<Workspace>> ws;
+    std::vector<int> node_of_pos;
 
     // mesh and operators (positions)
     DBuf<double> V, area, gb, larea, g11, g12, g22, lval, mass;
@@ -88,35 +114,36 @@ struct fcw_ctx {
     DBuf<fcw::LevelTile> fwd_tiles, bwd_tiles;
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
     DBuf<double> panel_heat, panel_pois;
-    // block work arrays
-    DBuf<double> X, Y, Z, U, Dt, Dout;
-    DBuf<int> src_pos;
+    DBuf<double> Dout;
+    // per-call plans
+    DBuf<fcw::LevelTile> plan_tiles;
+    DBuf<int> plan_nodes;
+    DBuf<unsigned char> plan_flags;
     // metrics
     bool metrics_ready = false;
-    int S = 0, q = 0, nb = 0, band_cap = 16384;
+    int S = 0, q = 0, nb = 0, band_cap = 6144;
     double eps = 1e-6, health_h = 0.0, width = 0.0, cap = -1.0, rtol = 1e-10;
     int max_iter = 60;
     DBuf<double> targets, fracs, vw;
-    DBuf<int> boundary, band;
+    DBuf<int> boundary;
     DBuf<double> o_scal, o_radii, o_perim, o_er, o_ea;
     DBuf<float> o_sr, o_sa;
     size_t o_cap = 0;
     // timing
     double stage[5] = {0, 0, 0, 0, 0};
     bool profiling = false;
-    double ktime[3] = {0, 0, 0};
-    int64_t kcount[3] = {0, 0, 0};
-    std::vector<EventPair> ev_pool;
-    std::vector<std::pair<int, EventPair>> ev_pending;
-    size_t ev_used = 0;
+    double ktime[kNumTimed] = {0, 0, 0};
+    int64_t kcount[kNumTimed] = {0, 0, 0};
 
     ~fcw_ctx() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto& e : ev_pool) {
-            (void)hipEventDestroy(e.a);
-            (void)hipEventDestroy(e.b);
+        for (auto& w : ws) {
+            if (w->stream) (void)hipStreamSynchronize(w->stream);
+            for (auto& e : w->ev_pool) {
+                (void)hipEventDestroy(e.a);
+                (void)hipEventDestroy(e.b);
+            }
+            if (w->stream) (void)hipStreamDestroy(w->stream);
         }
-        if (stream) (void)hipStreamDestroy(stream);
     }
 
     fcw::MeshDev mesh_dev() const {
@@ -136,89 +163,150 @@ struct fcw_ctx {
         return md;
     }
 
-    EventPair* ev_begin(int which) {
+    EventPair* ev_begin(Workspace& w, int which) {
         if (!profiling) return nullptr;
-        if (ev_used == ev_pool.size()) {
+        if (w.ev_used == w.ev_pool.size()) {
             EventPair e;
             HIPCHK(hipEventCreate(&e.a));
             HIPCHK(hipEventCreate(&e.b));
-            ev_pool.push_back(e);
+            w.ev_pool.push_back(e);
         }
-        EventPair* e = &ev_pool[ev_used++];
-        HIPCHK(hipEventRecord(e->a, stream));
-        ev_pending.push_back({which, *e});
+        EventPair* e = &w.ev_pool[w.ev_used++];
+        HIPCHK(hipEventRecord(e->a, w.stream));
+        w.ev_pending.push_back({which, *e});
         return e;
     }
-    void ev_end(EventPair* e) {
-        if (e) HIPCHK(hipEventRecord(e->b, stream));
+    void ev_end(Workspace& w, EventPair* e) {
+        if (e) HIPCHK(hipEventRecord(e->b, w.stream));
     }
-    void ev_collect() {
-        if (ev_pending.empty()) return;
-        HIPCHK(hipStreamSynchronize(stream));
-        for (auto& pe : ev_pending) {
+    void ev_collect(Workspace& w) {
+        if (w.ev_pending.empty()) return;
+        HIPCHK(hipStreamSynchronize(w.stream));
+        for (auto& pe : w.ev_pending) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, pe.second.a, pe.second.b));
             ktime[pe.first] += ms;
             kcount[pe.first] += 1;
         }
-        ev_pending.clear();
-        ev_used = 0;
+        w.ev_pending.clear();
+        w.ev_used = 0;
+    }
+
+    void alloc_workspace(Workspace& w) {
+        const size_t ld = (size_t)ldk;
+        HIPCHK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+        w.X.alloc((size_t)n * ld, dev_bytes);
+        w.Y.alloc((size_t)n * ld, dev_bytes);
+        w.Z.alloc((size_t)n * ld, dev_bytes);
+        w.U.alloc(std::max<size_t>(1, (size_t)sym.urows * ld), dev_bytes);
+        w.Dt.alloc((size_t)n * ld, dev_bytes);
+        w.src_pos.alloc(ld, dev_bytes);
     }
 
     // One system: forward then backward sweep. X (RHS, overwritten) -> Z (solution), Y scratch.
-    void solve(const double* panel, double* Xb, double* Yb, double* Zb) {
+    // With a plan, the forward sweep visits only the plan's active supernodes; every other
+    // node's forward result is zero (Y is cleared first).
+    void solve(Workspace& w, const double* panel, const BlockPlan* plan) {
         const int L = sym.nlevels;
+        if (plan) fcw::launch_fill(w.stream, w.Y.p, (int64_t)n * ldk, 0.0);
+        const unsigned char* flags = plan ? plan_flags.p + plan->flag_off : nullptr;
         for (int l = 0; l < L; ++l) {
+            const fcw::LevelTile* tiles;
+            int ntiles;
+            const int* nodes;
+            int nnodes;
+            if (plan) {
+                tiles = plan_tiles.p + plan->tile_off[l];
+                ntiles = plan->tile_off[l + 1] - plan->tile_off[l];
+                nodes = plan_nodes.p + plan->node_off[l];
+                nnodes = plan->node_off[l + 1] - plan->node_off[l];
+            } else {
+                tiles = fwd_tiles.p + fwd_tile_off[l];
+                ntiles = fwd_tile_off[l + 1] - fwd_tile_off[l];
+                nodes = level_nodes.p + level_off[l];
+                nnodes = level_off[l + 1] - level_off[l];
+            }
             if (l > 0)
-                fcw::launch_fwd_assemble(stream, level_nodes.p + level_off[l], level_off[l + 1] - level_off[l], ldk,
-                                         node_start.p, node_s.p, node_b.p, node_boff.p, node_uoff.p, child_ptr.p,
-                                         child_list.p, cmap.p, Xb, U.p);
-            EventPair* e = ev_begin(0);
-            fcw::launch_fwd_gemm(stream, fwd_tiles.p + fwd_tile_off[l], fwd_tile_off[l + 1] - fwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, panel, Xb, Yb, U.p,
-                                 l > 0);
-            ev_end(e);
+                fcw::launch_fwd_assemble(w.stream, nodes, nnodes, ldk, node_start.p, node_s.p, node_b.p, node_boff.p,
+                                         node_uoff.p, child_ptr.p, child_list.p, cmap.p, flags, w.X.p, w.U.p);
+            EventPair* e = ev_begin(w, kFwdGemm);
+            fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
+                                 node_uoff.p, panel, w.X.p, w.Y.p, w.U.p, l > 0);
+            ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
-            EventPair* e = ev_begin(1);
-            fcw::launch_bwd_gemm(stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Yb, Zb);
-            ev_end(e);
+            EventPair* e = ev_begin(w, kBwdGemm);
+            fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
+                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y.p,
+                                 w.Z.p);
+            ev_end(w, e);
         }
     }
 
-    // Distances for the k sources whose positions are already in src_pos[0:k]; result in Dt (k x n).
-    void distances(int k, double* st_times) {
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-        if (st_times) {
-            HIPCHK(hipEventCreate(&e0));
-            HIPCHK(hipEventCreate(&e1));
-            HIPCHK(hipEventCreate(&e2));
-            HIPCHK(hipEventCreate(&e3));
-            HIPCHK(hipEventRecord(e0, stream));
+    // Distances for the k sources whose positions are in w.src_pos[0:k]; result in w.Dt (k x n).
+    void distances(Workspace& w, int k, const BlockPlan* plan) {
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        if (profiling) {
+            for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+            HIPCHK(hipEventRecord(ev[0], w.stream));
         }
-        fcw::launch_delta_rhs(stream, X.p, n, ldk, src_pos.p, k);
-        solve(panel_heat.p, X.p, Y.p, Z.p);
-        if (st_times) HIPCHK(hipEventRecord(e1, stream));
-        fcw::launch_divergence(stream, mesh_dev(), ldk, Z.p, X.p);
-        if (st_times) HIPCHK(hipEventRecord(e2, stream));
-        solve(panel_pois.p, X.p, Y.p, Z.p);
-        fcw::launch_shift_transpose(stream, n, ldk, k, Z.p, src_pos.p, Dt.p);
-        if (st_times) {
-            HIPCHK(hipEventRecord(e3, stream));
-            HIPCHK(hipEventSynchronize(e3));
-            float a = 0, b = 0, c = 0;
-            HIPCHK(hipEventElapsedTime(&a, e0, e1));
-            HIPCHK(hipEventElapsedTime(&b, e1, e2));
-            HIPCHK(hipEventElapsedTime(&c, e2, e3));
-            st_times[0] += a * 1e-3;
-            st_times[1] += b * 1e-3;
-            st_times[2] += c * 1e-3;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            (void)hipEventDestroy(e2);
-            (void)hipEventDestroy(e3);
+        fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
+        solve(w, panel_heat.p, plan);
+        if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
+        fcw::launch_divergence(w.stream, mesh_dev(), ldk, w.Z.p, w.X.p);
+        if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
+        solve(w, panel_pois.p, nullptr);
+        fcw::launch_shift_transpose(w.stream, n, ldk, k, w.Z.p, w.src_pos.p, w.Dt.p);
+        if (profiling) {
+            HIPCHK(hipEventRecord(ev[3], w.stream));
+            HIPCHK(hipEventSynchronize(ev[3]));
+            for (int i = 0; i < 3; ++i) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                stage[i] += ms * 1e-3;
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e);
         }
+    }
+
+    // Host plans for consecutive blocks of `pos` (source positions), uploaded once.
+    std::vector<BlockPlan> make_plans(const std::vector<int>& pos) {
+        const int L = sym.nlevels, nn = sym.nnodes;
+        const int nblocks = ((int)pos.size() + ldk - 1) / ldk;
+        std::vector<BlockPlan> plans(nblocks);
+        std::vector<fcw::LevelTile> tiles;
+        std::vector<int> nodes;
+        std::vector<unsigned char> flags((size_t)nblocks * nn, 0);
+        std::vector<std::vector<int>> per_level(L);
+        for (int b = 0; b < nblocks; ++b) {
+            unsigned char* fl = flags.data() + (size_t)b * nn;
+            for (auto& v : per_level) v.clear();
+            const int j1 = std::min((int)pos.size(), (b + 1) * ldk);
+            for (int j = b * ldk; j < j1; ++j) {
+                for (int id = node_of_pos[pos[j]]; id >= 0 && !fl[id]; id = sym.node_parent[id]) {
+                    fl[id] = 1;
+                    per_level[sym.node_level[id]].push_back(id);
+                }
+            }
+            BlockPlan& P = plans[b];
+            P.flag_off = (size_t)b * nn;
+            P.tile_off.push_back((int)tiles.size());
+            P.node_off.push_back((int)nodes.size());
+            for (int l = 0; l < L; ++l) {
+                std::sort(per_level[l].begin(), per_level[l].end());
+                for (int id : per_level[l]) {
+                    nodes.push_back(id);
+                    const int f = sym.node_s[id] + sym.node_b[id];
+                    for (int m0 = 0; m0 < f; m0 += fcw::kTileM) tiles.push_back({id, m0});
+                }
+                P.tile_off.push_back((int)tiles.size());
+                P.node_off.push_back((int)nodes.size());
+            }
+        }
+        plan_tiles.upload(tiles.data(), std::max<size_t>(1, tiles.size()), dev_bytes);
+        plan_nodes.upload(nodes.data(), std::max<size_t>(1, nodes.size()), dev_bytes);
+        plan_flags.upload(flags.data(), std::max<size_t>(1, flags.size()), dev_bytes);
+        return plans;
     }
 
     void ensure_outputs(size_t k) {
@@ -234,7 +322,7 @@ struct fcw_ctx {
         o_cap = k;
     }
 
-    fcw::MetricsDev metrics_dev() {
+    fcw::MetricsDev metrics_dev(Workspace& w) {
         fcw::MetricsDev md;
         md.S = S;
         md.q = q;
@@ -250,7 +338,7 @@ struct fcw_ctx {
         md.rtol = rtol;
         md.max_iter = max_iter;
         md.band_cap = band_cap;
-        md.band = band.p;
+        md.band = w.band.p;
         return md;
     }
 
@@ -267,10 +355,15 @@ struct fcw_ctx {
         return o;
     }
 
+    void sync_all() {
+        for (auto& w : ws) HIPCHK(hipStreamSynchronize(w->stream));
+    }
+
     void copy_outputs(size_t k, fcw_metrics_out* out) {
+        sync_all();
         const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
         auto cp = [&](void* dst, const void* src, size_t bytes) {
-            if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+            if (dst && bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         };
         cp(out->scalars, o_scal.p, k * fcw::kSlotCount * sizeof(double));
         cp(out->radii, o_radii.p, k * S * sizeof(double));
@@ -279,7 +372,6 @@ struct fcw_ctx {
         cp(out->extra_a, o_ea.p, k * NE * sizeof(double));
         cp(out->sample_r, o_sr.p, k * (S + NE) * sizeof(float));
         cp(out->sample_a, o_sa.p, k * (S + NE) * sizeof(float));
-        HIPCHK(hipStreamSynchronize(stream));
     }
 };
 
@@ -382,6 +474,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->host_threads = 0;
     opt->device = 0;
     opt->ir_steps = 2;
+    opt->streams = 2;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -409,6 +502,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         if (opt.precision != FCW_FP64 && opt.precision != FCW_FP32_IR) throw Fail("unknown precision");
         if (opt.precision == FCW_FP32_IR) throw Fail("FCW_FP32_IR is not available in this build");
         if (opt.block <= 0) opt.block = 256;
+        if (opt.streams <= 0) opt.streams = 1;
+        if (opt.streams > 8) throw Fail("streams must be in 1..8");
         if (!(opt.poisson_shift > 0.0) || !std::isfinite(opt.poisson_shift))
             throw Fail("poisson_shift must be positive and finite");
         check_faces(n, Vin, m, F);
@@ -418,13 +513,14 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->m = m;
         C->ldk = ((opt.block + fcw::kColPad - 1) / fcw::kColPad) * fcw::kColPad;
         HIPCHK(hipSetDevice(opt.device));
-        HIPCHK(hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking));
         double t0 = now_s();
         std::string err;
         fcw::Symbolic& S = C->sym;
         if (fcw::analyse(n, Vin, m, F, opt.leaf_size > 0 ? opt.leaf_size : 64, S, err)) throw Fail(err);
+        C->node_of_pos.assign(n, -1);
+        for (int id = 0; id < S.nnodes; ++id)
+            for (int p = S.node_start[id]; p < S.node_start[id] + S.node_s[id]; ++p) C->node_of_pos[p] = id;
         double t1 = now_s();
-        // positions
         std::vector<double> Vp((size_t)n * 3);
         for (int p = 0; p < n; ++p)
             for (int c = 0; c < 3; ++c) Vp[3 * (size_t)p + c] = Vin[3 * (size_t)S.perm[p] + c];
@@ -432,6 +528,11 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         const double t = (opt.t > 0.0) ? opt.t : opt.t_coef * mean_edge * mean_edge;
         if (!(t > 0.0) || !std::isfinite(t)) throw Fail("diffusion time must be positive and finite");
         int64_t& tb = C->dev_bytes;
+        for (int i = 0; i < opt.streams; ++i) {
+            C->ws.emplace_back(new Workspace());
+            C->alloc_workspace(*C->ws.back());
+        }
+        hipStream_t st = C->ws[0]->stream;
         C->V.upload(Vp.data(), Vp.size(), tb);
         C->faces.upload(S.faces.data(), S.faces.size(), tb);
         C->vf_ptr.upload(S.vf_ptr.data(), S.vf_ptr.size(), tb);
@@ -450,24 +551,24 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         const int nnz = S.rowptr[n];
         C->lval.alloc(nnz, tb);
         C->mass.alloc(n, tb);
-        fcw::launch_face_geometry(C->stream, m, C->V.p, C->faces.p, C->area.p, C->gb.p, C->larea.p, C->g11.p,
-                                  C->g12.p, C->g22.p);
-        fcw::launch_assemble(C->stream, n, nnz, C->cptr.p, C->contrib.p, C->area.p, C->gb.p, C->vf_ptr.p, C->vf.p,
+        fcw::launch_face_geometry(st, m, C->V.p, C->faces.p, C->area.p, C->gb.p, C->larea.p, C->g11.p, C->g12.p,
+                                  C->g22.p);
+        fcw::launch_assemble(st, n, nnz, C->cptr.p, C->contrib.p, C->area.p, C->gb.p, C->vf_ptr.p, C->vf.p,
                              C->lval.p, C->mass.p);
         HIPCHK(hipGetLastError());
         std::vector<double> lval(nnz), mass(n);
-        HIPCHK(hipMemcpyAsync(lval.data(), C->lval.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, C->stream));
-        HIPCHK(hipMemcpyAsync(mass.data(), C->mass.p, sizeof(double) * n, hipMemcpyDeviceToHost, C->stream));
-        HIPCHK(hipStreamSynchronize(C->stream));
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(lval.data(), C->lval.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mass.data(), C->mass.p, sizeof(double) * n, hipMemcpyDeviceToHost));
         double t2 = now_s();
         std::vector<double> ph((size_t)S.panel_elems), pp((size_t)S.panel_elems);
-        if (fcw::factor(S, lval.data(), mass.data(), t, 1.0, ph.data(), err, opt.host_threads)) throw Fail("heat: " + err);
+        if (fcw::factor(S, lval.data(), mass.data(), t, 1.0, ph.data(), err, opt.host_threads))
+            throw Fail("heat: " + err);
         if (fcw::factor(S, lval.data(), mass.data(), 1.0, opt.poisson_shift, pp.data(), err, opt.host_threads))
             throw Fail("poisson: " + err);
         double t3 = now_s();
         C->panel_heat.upload(ph.data(), ph.size(), tb);
         C->panel_pois.upload(pp.data(), pp.size(), tb);
-        // tree arrays and level tile lists
         C->node_start.upload(S.node_start.data(), S.node_start.size(), tb);
         C->node_s.upload(S.node_s.data(), S.node_s.size(), tb);
         C->node_b.upload(S.node_b.data(), S.node_b.size(), tb);
@@ -495,13 +596,6 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         }
         C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
         C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
-        const size_t ld = (size_t)C->ldk;
-        C->X.alloc((size_t)n * ld, tb);
-        C->Y.alloc((size_t)n * ld, tb);
-        C->Z.alloc((size_t)n * ld, tb);
-        C->U.alloc(std::max<size_t>(1, (size_t)S.urows * ld), tb);
-        C->Dt.alloc((size_t)n * ld, tb);
-        C->src_pos.alloc(ld, tb);
         double t4 = now_s();
         fcw_info& I = C->info;
         I.n = n;
@@ -536,8 +630,7 @@ int fcw_get_info(const fcw_ctx* ctx, fcw_info* out) {
     });
 }
 
-static void upload_sources(fcw_ctx* C, const int* src, int k) {
-    if (k <= 0 || k > C->ldk) throw Fail("source count must be in 1..block (" + std::to_string(C->ldk) + ")");
+static std::vector<int> source_positions(fcw_ctx* C, const int* src, int k) {
     std::vector<int> pos(k);
     for (int j = 0; j < k; ++j) {
         if (src[j] < 0 || src[j] >= C->n)
@@ -545,21 +638,24 @@ static void upload_sources(fcw_ctx* C, const int* src, int k) {
                        std::to_string(C->n) + " vertices");
         pos[j] = C->sym.iperm[src[j]];
     }
-    HIPCHK(hipMemcpyAsync(C->src_pos.p, pos.data(), sizeof(int) * k, hipMemcpyHostToDevice, C->stream));
-    HIPCHK(hipStreamSynchronize(C->stream));
+    return pos;
 }
 
 int fcw_distance_block(fcw_ctx* C, const int* src, int k, double* out) {
     return guard([&] {
         if (!C || !src || !out) throw Fail("null argument");
-        upload_sources(C, src, k);
-        C->distances(k, C->profiling ? C->stage : nullptr);
+        if (k <= 0 || k > C->ldk) throw Fail("source count must be in 1..block (" + std::to_string(C->ldk) + ")");
+        std::vector<int> pos = source_positions(C, src, k);
+        Workspace& w = *C->ws[0];
+        HIPCHK(hipMemcpy(w.src_pos.p, pos.data(), sizeof(int) * k, hipMemcpyHostToDevice));
+        std::vector<BlockPlan> plans = C->make_plans(pos);
+        C->distances(w, k, &plans[0]);
         if (!C->Dout.p) C->Dout.alloc((size_t)C->n * C->ldk, C->dev_bytes);
-        fcw::launch_gather_rows(C->stream, C->n, k, C->Dt.p, C->iperm.p, C->Dout.p);
+        fcw::launch_gather_rows(w.stream, C->n, k, w.Dt.p, C->iperm.p, C->Dout.p);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(out, C->Dout.p, sizeof(double) * (size_t)k * C->n, hipMemcpyDeviceToHost, C->stream));
-        HIPCHK(hipStreamSynchronize(C->stream));
-        C->ev_collect();
+        HIPCHK(hipStreamSynchronize(w.stream));
+        HIPCHK(hipMemcpy(out, C->Dout.p, sizeof(double) * (size_t)k * C->n, hipMemcpyDeviceToHost));
+        C->ev_collect(w);
     });
 }
 
@@ -577,6 +673,7 @@ int fcw_set_metrics(fcw_ctx* C, const double* targets, int S, const double* frac
         if (!(bin_width > 0.0) || !std::isfinite(bin_width)) throw Fail("bin_width must be positive");
         if (max_iter < 1) throw Fail("max_iter must be >= 1");
         int64_t& tb = C->dev_bytes;
+        C->sync_all();
         C->S = S;
         C->q = q;
         C->nb = nb;
@@ -590,16 +687,16 @@ int fcw_set_metrics(fcw_ctx* C, const double* targets, int S, const double* frac
             bpos[i] = C->sym.iperm[boundary[i]];
         }
         C->boundary.upload(bpos.data(), bpos.size(), tb);
-        std::vector<double> w(C->n);
-        for (int v = 0; v < C->n; ++v) w[C->sym.iperm[v]] = vertex_w[v];
-        C->vw.upload(w.data(), w.size(), tb);
+        std::vector<double> wv(C->n);
+        for (int v = 0; v < C->n; ++v) wv[C->sym.iperm[v]] = vertex_w[v];
+        C->vw.upload(wv.data(), wv.size(), tb);
         C->eps = eps;
         C->health_h = health_h;
         C->width = bin_width;
         C->cap = cap_fraction;
         C->rtol = rtol;
         C->max_iter = max_iter;
-        C->band.alloc((size_t)C->ldk * S * C->band_cap, tb);
+        for (auto& w : C->ws) w->band.alloc((size_t)C->ldk * S * 5 * C->band_cap, tb);
         C->o_cap = 0;
         C->metrics_ready = true;
     });
@@ -611,46 +708,43 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
         if (!C->metrics_ready) throw Fail("fcw_set_metrics must be called first");
         if (nsrc <= 0) return;
         const double w0 = now_s();
-        std::vector<int> pos(nsrc);
-        for (int j = 0; j < nsrc; ++j) {
-            if (src[j] < 0 || src[j] >= C->n)
-                throw Fail("source_idx out of range: " + std::to_string(src[j]) + " for mesh with " +
-                           std::to_string(C->n) + " vertices");
-            pos[j] = C->sym.iperm[src[j]];
-        }
+        std::vector<int> pos = source_positions(C, src, nsrc);
         DBuf<int> all_pos;
-        int64_t dummy = 0;
-        all_pos.upload(pos.data(), pos.size(), dummy);
+        int64_t scratch_bytes = 0;
+        all_pos.upload(pos.data(), pos.size(), scratch_bytes);
+        std::vector<BlockPlan> plans = C->make_plans(pos);
         C->ensure_outputs((size_t)nsrc);
         const fcw::MeshDev mdv = C->mesh_dev();
-        const fcw::MetricsDev met = C->metrics_dev();
-        for (int b0 = 0; b0 < nsrc; b0 += C->ldk) {
+        const int NS = (int)C->ws.size();
+        for (int b0 = 0, bi = 0; b0 < nsrc; b0 += C->ldk, ++bi) {
+            Workspace& w = *C->ws[bi % NS];
             const int k = std::min(C->ldk, nsrc - b0);
-            HIPCHK(hipMemcpyAsync(C->src_pos.p, all_pos.p + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, C->stream));
-            C->distances(k, C->profiling ? C->stage : nullptr);
-            hipEvent_t ea = nullptr, eb = nullptr;
-            EventPair* e = C->ev_begin(2);
+            HIPCHK(hipMemcpyAsync(w.src_pos.p, all_pos.p + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, w.stream));
+            C->distances(w, k, &plans[bi]);
+            EventPair* e = C->ev_begin(w, kMetrics);
+            hipEvent_t ma = nullptr, mb = nullptr;
             if (C->profiling) {
-                HIPCHK(hipEventCreate(&ea));
-                HIPCHK(hipEventCreate(&eb));
-                HIPCHK(hipEventRecord(ea, C->stream));
+                HIPCHK(hipEventCreate(&ma));
+                HIPCHK(hipEventCreate(&mb));
+                HIPCHK(hipEventRecord(ma, w.stream));
             }
-            fcw::launch_metrics(C->stream, k, mdv, met, C->Dt.p, C->src_pos.p, C->outputs_at((size_t)b0), true);
-            C->ev_end(e);
-            if (C->profiling) {
-                HIPCHK(hipEventRecord(eb, C->stream));
-                HIPCHK(hipEventSynchronize(eb));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ea, eb));
-                C->stage[3] += ms * 1e-3;
-                (void)hipEventDestroy(ea);
-                (void)hipEventDestroy(eb);
-                C->ev_collect();
-            }
+            fcw::launch_metrics(w.stream, k, mdv, C->metrics_dev(w), w.Dt.p, w.src_pos.p, C->outputs_at((size_t)b0),
+                                true);
+            C->ev_end(w, e);
             HIPCHK(hipGetLastError());
+            if (C->profiling) {
+                HIPCHK(hipEventRecord(mb, w.stream));
+                HIPCHK(hipEventSynchronize(mb));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ma, mb));
+                C->stage[3] += ms * 1e-3;
+                (void)hipEventDestroy(ma);
+                (void)hipEventDestroy(mb);
+                C->ev_collect(w);
+            }
         }
         C->copy_outputs((size_t)nsrc, out);
-        C->ev_collect();
+        for (auto& w : C->ws) C->ev_collect(*w);
         C->stage[4] = now_s() - w0;
     });
 }
@@ -662,14 +756,16 @@ int fcw_geometry_fields(fcw_ctx* C, const double* D, const int* src, int k, int 
         if (!C->metrics_ready) throw Fail("fcw_set_metrics must be called first");
         if (k <= 0 || k > C->ldk) throw Fail("field count must be in 1..block");
         std::vector<int> zeros(k, 0);
-        upload_sources(C, src ? src : zeros.data(), k);
-        // fields to positions: Dt[j][p] = D[j][perm[p]]
+        std::vector<int> pos = source_positions(C, src ? src : zeros.data(), k);
+        Workspace& w = *C->ws[0];
+        C->sync_all();
+        HIPCHK(hipMemcpy(w.src_pos.p, pos.data(), sizeof(int) * k, hipMemcpyHostToDevice));
         std::vector<double> Dp((size_t)k * C->n);
         for (int j = 0; j < k; ++j)
             for (int p = 0; p < C->n; ++p) Dp[(size_t)j * C->n + p] = D[(size_t)j * C->n + C->sym.perm[p]];
-        HIPCHK(hipMemcpyAsync(C->Dt.p, Dp.data(), sizeof(double) * Dp.size(), hipMemcpyHostToDevice, C->stream));
+        HIPCHK(hipMemcpy(w.Dt.p, Dp.data(), sizeof(double) * Dp.size(), hipMemcpyHostToDevice));
         C->ensure_outputs((size_t)k);
-        fcw::launch_metrics(C->stream, k, C->mesh_dev(), C->metrics_dev(), C->Dt.p, C->src_pos.p, C->outputs_at(0),
+        fcw::launch_metrics(w.stream, k, C->mesh_dev(), C->metrics_dev(w), w.Dt.p, w.src_pos.p, C->outputs_at(0),
                             health_screen != 0);
         HIPCHK(hipGetLastError());
         C->copy_outputs((size_t)k, out);
@@ -686,8 +782,9 @@ int fcw_stage_times(fcw_ctx* C, double* out, int len) {
 int fcw_set_profiling(fcw_ctx* C, int on) {
     return guard([&] {
         if (!C) throw Fail("null context");
+        C->sync_all();
         C->profiling = on != 0;
-        for (int i = 0; i < 3; ++i) {
+        for (int i = 0; i < kNumTimed; ++i) {
             C->ktime[i] = 0.0;
             C->kcount[i] = 0;
         }
@@ -698,7 +795,7 @@ int fcw_set_profiling(fcw_ctx* C, int on) {
 int fcw_kernel_times(fcw_ctx* C, double* out, int len) {
     if (!C || !out) return 0;
     int c = 0;
-    for (int i = 0; i < 3 && c + 1 < len + 1; ++i) {
+    for (int i = 0; i < kNumTimed; ++i) {
         if (c < len) out[c++] = C->ktime[i];
         if (c < len) out[c++] = (double)C->kcount[i];
     }
@@ -715,7 +812,6 @@ int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_
         std::vector<std::vector<std::pair<int, double>>> rows;
         std::vector<double> mass;
         host_operators(n, V, m, F, rows, mass);
-        // values in S's CSR (positions)
         std::vector<double> lval(S.rowptr[n], 0.0), mpos(n);
         for (int v = 0; v < n; ++v) {
             const int p = S.iperm[v];
@@ -723,8 +819,8 @@ int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_
             const int* row = S.colind.data() + S.rowptr[p];
             const int len = S.rowptr[p + 1] - S.rowptr[p];
             for (auto& e : rows[v]) {
-                const int q = S.iperm[e.first];
-                const int pos = (int)(std::lower_bound(row, row + len, q) - row);
+                const int qv = S.iperm[e.first];
+                const int pos = (int)(std::lower_bound(row, row + len, qv) - row);
                 lval[S.rowptr[p] + pos] += e.second;
             }
         }

@@ -22,7 +22,7 @@ constexpr int kColPad = 64;  // RHS block widths are padded to this many columns
 constexpr int kMaxScales = 32;
 constexpr int kMaxExtras = 256;
 constexpr int kMaxBins = 2048;
-constexpr int kMetricThreads = 1024;
+constexpr int kMetricThreads = 512;
 
 // Per-source scalar outputs, in this order (double each).
 enum ScalarSlot {
@@ -61,8 +61,8 @@ struct MetricsDev {
     double eps, health_h, width, cap;  // cap < 0: no cap
     double rtol;
     int max_iter;
-    int band_cap;           // entries per scale per source in the band scratch
-    int* band;              // k x S x band_cap
+    int band_cap;           // band faces per scale per source held in the scratch
+    double* band;           // k x S x 5 x band_cap (sorted a, b, c, area, |grad d| per band face)
 };
 
 struct MetricsOut {
@@ -83,7 +83,8 @@ void launch_fill(hipStream_t st, double* p, int64_t count, double value);
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);
 void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
                          const int* node_s, const int* node_b, const int64_t* node_boff, const int64_t* node_uoff,
-                         const int* child_ptr, const int* child_list, const int* cmap, double* X, double* U);
+                         const int* child_ptr, const int* child_list, const int* cmap, const unsigned char* active,
+                         double* X, double* U);
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const double* panel, const double* X, double* Y, double* U, bool accumulate);

@@ -35,9 +35,11 @@ __all__ = [
     "TWELVE_SCALES",
 ]
 
-# The 12 log-spaced area fractions 0.002 .. 0.05 used by the reference's README,
-# run_pipeline.py:248-267 and test_fastcw_lean.py:51-52 (= 0.002 * 25**(i/11)).
-TWELVE_SCALES = tuple(float(0.002 * 25.0 ** (i / 11.0)) for i in range(12))
+# The 12 log-spaced area fractions 0.002 .. 0.05 (0.002 * 25**(i/11) rounded to 8 decimals),
+# as the reference lists them: run_pipeline.py:248-267, verify_lean_integration.py:139-140,
+# test_fastcw_lean.py:51-52.
+TWELVE_SCALES = (0.002, 0.00267988, 0.00359088, 0.00481157, 0.00644721, 0.00863888,
+                 0.01157558, 0.01551059, 0.02078326, 0.02784833, 0.03731509, 0.05)
 
 
 def icosphere(level: int):

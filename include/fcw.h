@@ -50,7 +50,8 @@ typedef struct {
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
     int ir_steps;          /* refinement sweeps per system in FCW_FP32_IR mode; default 2 */
-    int reserved[6];
+    int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 2 */
+    int reserved[5];
 } fcw_options;
 
 typedef struct {
