@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-sources", type=int, default=24)
     ap.add_argument("--profile-blocks", type=int, default=4)
+    ap.add_argument("--calibrate-bytes", type=int, default=0,
+                    help="run one plain copy kernel of this many bytes first (byte-counter calibration)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
@@ -220,6 +222,8 @@ def main():
     log(f"bench: {prep['V'].shape[0]} cortical vertices, {prep['F'].shape[0]} faces")
     from fastcorticalwiring_amd import _lib
 
+    if args.calibrate_bytes:
+        _lib.check(_lib.lib().fcw_calibration_copy(int(args.calibrate_bytes), 0))
     forms = ["fp64", "fp32_ir"] if args.precision == "auto" else [args.precision]
     results = {}
     for form in forms:

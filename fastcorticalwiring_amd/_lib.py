@@ -79,6 +79,7 @@ EXPORTED_SYMBOLS = (
     "fcw_default_options", "fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block",
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
+    "fcw_calibration_copy",
 )
 
 _lib = None
@@ -115,11 +116,12 @@ def lib():
     L.fcw_build_info.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.fcw_last_error.argtypes = []
     L.fcw_last_error.restype = ctypes.c_char_p
+    L.fcw_calibration_copy.argtypes = [ctypes.c_int64, ctypes.c_int]
     L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
-                 "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve"):
+                 "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

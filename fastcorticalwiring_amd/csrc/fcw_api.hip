@@ -802,6 +802,22 @@ int fcw_kernel_times(fcw_ctx* C, double* out, int len) {
     return c;
 }
 
+int fcw_calibration_copy(int64_t bytes, int device) {
+    return guard([&] {
+        if (bytes <= 0 || bytes % 32) throw Fail("bytes must be a positive multiple of 32");
+        HIPCHK(hipSetDevice(device));
+        void *a = nullptr, *b = nullptr;
+        HIPCHK(hipMalloc(&a, (size_t)bytes));
+        HIPCHK(hipMalloc(&b, (size_t)bytes));
+        fcw::launch_calibration_copy(nullptr, a, b, 0);  // nothing copied: warm the code object
+        fcw::launch_calibration_copy(nullptr, a, b, bytes);
+        hipError_t e = hipDeviceSynchronize();
+        (void)hipFree(a);
+        (void)hipFree(b);
+        HIPCHK(e);
+    });
+}
+
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale, int leaf_size,
                           const double* B, int k, double* X, double* stats) {
     return guard([&] {

@@ -80,6 +80,7 @@ void launch_face_geometry(hipStream_t st, int m, const double* V, const int* fac
 void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, const int* contrib, const double* area,
                      const double* gb, const int* vf_ptr, const int* vf, double* lval, double* mass);
 void launch_fill(hipStream_t st, double* p, int64_t count, double value);
+void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes);
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);
 void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
                          const int* node_s, const int* node_b, const int64_t* node_boff, const int64_t* node_uoff,

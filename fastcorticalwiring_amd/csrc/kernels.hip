@@ -86,6 +86,11 @@ __global__ void fill_kernel(double* __restrict__ p, int64_t count, double value)
         p[i] = value;
 }
 
+__global__ void calibration_copy_kernel(const double4* __restrict__ src, double4* __restrict__ dst, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void delta_rhs_kernel(double* __restrict__ X, int ldk, const int* __restrict__ src_pos, int k) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < k) X[(int64_t)src_pos[j] * ldk + j] = 1.0;
@@ -124,131 +129,6 @@ __global__ __launch_bounds__(256) void fwd_assemble_kernel(
                 Ut[(int64_t)(t - s) * ldk + col] += v;
         }
         __syncthreads();
-    }
-}
-
-// [Y_S; U_S] (+)= P_S X_S : P_S is (s+b) x s column-major, X_S the node's s RHS rows.
-__global__ __launch_bounds__(256) void fwd_gemm_kernel(
-    const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
-    const int* __restrict__ node_b, const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff,
-    const double* __restrict__ panel, const double* __restrict__ X, double* __restrict__ Y, double* __restrict__ U,
-    int accumulate) {
-    const LevelTile t = tiles[blockIdx.x];
-    const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
-    const double* P = panel + node_poff[t.node];
-    const int n0 = blockIdx.y * kTileN;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    __shared__ double As[kTileK][kTileM + 1];
-    __shared__ double Bs[kTileK][kTileN];
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < s; k0 += kTileK) {
-#pragma unroll
-        for (int e = tid; e < kTileM * kTileK; e += 256) {
-            const int r = e % kTileM, kk = e / kTileM;
-            const int row = t.m0 + r, col = k0 + kk;
-            As[kk][r] = (row < f && col < s) ? P[row + (int64_t)col * f] : 0.0;
-        }
-#pragma unroll
-        for (int e = tid; e < kTileK * kTileN; e += 256) {
-            const int c = e % kTileN, kk = e / kTileN;
-            const int row = k0 + kk;
-            Bs[kk][c] = (row < s) ? X[(int64_t)(start + row) * ldk + n0 + c] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kTileK; ++kk) {
-            double a[4], bb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
-        }
-        __syncthreads();
-    }
-    double* Ut = U + node_uoff[t.node] * ldk;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = t.m0 + ty + 16 * i;
-        if (row >= f) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = n0 + tx + 16 * j;
-            if (row < s) {
-                Y[(int64_t)(start + row) * ldk + col] = acc[i][j];
-            } else {
-                double* u = Ut + (int64_t)(row - s) * ldk + col;
-                *u = accumulate ? (*u + acc[i][j]) : acc[i][j];
-            }
-        }
-    }
-}
-
-// Z_S = P_S^T [Y_S; -Z_B] : B = the node's boundary rows (ancestors, already solved).
-__global__ __launch_bounds__(256) void bwd_gemm_kernel(
-    const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
-    const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
-    const int64_t* __restrict__ node_poff, const double* __restrict__ panel, const double* __restrict__ Y,
-    double* __restrict__ Z) {
-    const LevelTile t = tiles[blockIdx.x];
-    const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
-    const double* P = panel + node_poff[t.node];
-    const int* B = bidx + node_boff[t.node];
-    const int n0 = blockIdx.y * kTileN;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    __shared__ double As[kTileK][kTileM + 1];
-    __shared__ double Bs[kTileK][kTileN];
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-    for (int k0 = 0; k0 < f; k0 += kTileK) {
-#pragma unroll
-        for (int e = tid; e < kTileM * kTileK; e += 256) {
-            const int kk = e % kTileK, r = e / kTileK;
-            const int orow = t.m0 + r, l = k0 + kk;
-            As[kk][r] = (orow < s && l < f) ? P[l + (int64_t)orow * f] : 0.0;
-        }
-#pragma unroll
-        for (int e = tid; e < kTileK * kTileN; e += 256) {
-            const int c = e % kTileN, kk = e / kTileN;
-            const int l = k0 + kk;
-            double v = 0.0;
-            if (l < s)
-                v = Y[(int64_t)(start + l) * ldk + n0 + c];
-            else if (l < f)
-                v = -Z[(int64_t)B[l - s] * ldk + n0 + c];
-            Bs[kk][c] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kTileK; ++kk) {
-            double a[4], bb[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], bb[j], acc[i][j]);
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = t.m0 + ty + 16 * i;
-        if (row >= s) continue;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) Z[(int64_t)(start + row) * ldk + n0 + tx + 16 * j] = acc[i][j];
     }
 }
 
@@ -336,6 +216,11 @@ void launch_fill(hipStream_t st, double* p, int64_t count, double value) {
     fill_kernel<<<dim3((unsigned)blocks), 256, 0, st>>>(p, count, value);
 }
 
+void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes) {
+    const int64_t count = bytes / (int64_t)sizeof(double4);
+    calibration_copy_kernel<<<dim3(4096), 256, 0, st>>>((const double4*)src, (double4*)dst, count);
+}
+
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k) {
     launch_fill(st, X, (int64_t)n * ldk, 0.0);
     delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
@@ -348,22 +233,6 @@ void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, c
     if (count <= 0) return;
     fwd_assemble_kernel<<<dim3(count, ldk / 64), 256, 0, st>>>(nodes, ldk, node_start, node_s, node_b, node_boff,
                                                                 node_uoff, child_ptr, child_list, cmap, active, X, U);
-}
-
-void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
-                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
-                     const double* panel, const double* X, double* Y, double* U, bool accumulate) {
-    if (ntiles <= 0) return;
-    fwd_gemm_kernel<<<dim3(ntiles, ldk / kTileN), 256, 0, st>>>(tiles, ldk, node_start, node_s, node_b, node_poff,
-                                                                 node_uoff, panel, X, Y, U, accumulate ? 1 : 0);
-}
-
-void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
-                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z) {
-    if (ntiles <= 0) return;
-    bwd_gemm_kernel<<<dim3(ntiles, ldk / kTileN), 256, 0, st>>>(tiles, ldk, node_start, node_s, node_b, node_boff,
-                                                                 bidx, node_poff, panel, Y, Z);
 }
 
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {

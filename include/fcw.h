@@ -128,6 +128,10 @@ int fcw_kernel_times(fcw_ctx* ctx, double* out, int len);
 int fcw_build_info(char* buf, int len);
 const char* fcw_last_error(void);
 
+/* One launch of a plain device copy of `bytes` bytes (reads `bytes`, writes `bytes`): the calibration
+ * point for hardware byte counters when profiling the other kernels. No reference counterpart. */
+int fcw_calibration_copy(int64_t bytes, int device);
+
 /* Host-only check of the analysis and factorisation (no GPU): solves A X = B for
  * A = l_scale*L + m_scale*M assembled on the host from (V, F), with the CPU application of the
  * partitioned-inverse panels. B/X are n x k row-major in input vertex order. stats[0..3] =

@@ -1,0 +1,109 @@
+"""Serving loop: admission and overlap logic on the CPU (fake contexts), and on the GPU the
+results of each hemisphere alone vs in a concurrent mix."""
+
+import threading
+import time
+
+import numpy as np
+import pytest
+
+
+class _FakeCtx:
+    live = 0
+    peak = 0
+    lock = threading.Lock()
+
+    def __init__(self, n):
+        self.block = 4
+        self.n = n
+        with _FakeCtx.lock:
+            _FakeCtx.live += 1
+            _FakeCtx.peak = max(_FakeCtx.peak, _FakeCtx.live)
+
+    def wiring(self, src):
+        time.sleep(0.002 * len(src) / self.block)
+        k = len(src)
+        return {"scalars": np.tile(np.asarray(src, float)[:, None], (1, 9)), "radii": np.zeros((k, 2))}
+
+    def close(self):
+        with _FakeCtx.lock:
+            _FakeCtx.live -= 1
+
+
+class _FakePrep:
+    def __init__(self, req):
+        n = int(np.count_nonzero(req.cortex_mask))
+        self.order = np.arange(n)[::-1].copy()
+        self.s2o = np.flatnonzero(req.cortex_mask)
+        self.ctx = _FakeCtx(n)
+
+
+def _fake_requests(k, n=40):
+    from fastcorticalwiring_amd.serving import HemisphereRequest
+
+    out = []
+    for i in range(k):
+        m = np.ones(n, bool)
+        m[: i % 5] = False
+        out.append(HemisphereRequest(i, 0.01 * i, np.zeros((n, 3)), np.zeros((1, 3), np.int32), m))
+    return out
+
+
+def test_admission_respects_slots_and_completes_everything():
+    from fastcorticalwiring_amd.serving import ServingEngine
+
+    _FakeCtx.live = _FakeCtx.peak = 0
+    eng = ServingEngine(hbm_budget_bytes=1 << 60, max_active=2, chunk_blocks=2, prepare=_FakePrep)
+    res = eng.run(_fake_requests(7))
+    assert all(r.error is None for r in res)
+    assert _FakeCtx.peak <= 2 and _FakeCtx.live == 0
+    for r in res:
+        assert r.done_s >= r.ready_s >= r.admitted_s >= r.arrival_s - 1e-9
+        assert np.array_equal(r.outputs["scalars"][:, 0], r.outputs["order"].astype(float))
+    s = ServingEngine.summary(res)
+    assert s["requests"] == 7 and s["failed"] == 0 and s["p50_s"] <= s["p95_s"] <= s["p99_s"]
+
+
+def test_admission_respects_memory_budget():
+    from fastcorticalwiring_amd.serving import ServingEngine, estimate_device_bytes
+
+    _FakeCtx.live = _FakeCtx.peak = 0
+    one = estimate_device_bytes(40, 80, 256)
+    eng = ServingEngine(hbm_budget_bytes=int(1.5 * one), max_active=8, chunk_blocks=1, prepare=_FakePrep)
+    res = eng.run(_fake_requests(5))
+    assert all(r.error is None for r in res)
+    assert _FakeCtx.peak == 1  # budget admits one context at a time
+
+
+def test_a_failing_request_does_not_stop_the_service():
+    from fastcorticalwiring_amd.serving import ServingEngine
+
+    def prep(req):
+        if req.rid == 1:
+            raise ValueError("broken mesh")
+        return _FakePrep(req)
+
+    eng = ServingEngine(max_active=2, prepare=prep)
+    res = eng.run(_fake_requests(3))
+    assert res[1].error and "broken mesh" in res[1].error
+    assert res[0].error is None and res[2].error is None
+
+
+@pytest.mark.gpu
+def test_results_alone_equal_results_in_the_mix():
+    from fastcorticalwiring_amd import meshes
+    from fastcorticalwiring_amd.serving import HemisphereRequest, ServingEngine
+
+    reqs = []
+    for i, (lvl, seed) in enumerate(((4, 11), (5, 12), (4, 13))):
+        s = meshes.make_surface(lvl, 70.0, 0.15, 16, 0.3, 0.1, seed)
+        reqs.append(HemisphereRequest(i, 0.0, s.vertices, s.faces, s.cortex_mask))
+    alone = [ServingEngine(max_active=1).run([r])[0] for r in reqs]
+    mix = ServingEngine(max_active=3, chunk_blocks=1).run(reqs)
+    for a, b in zip(alone, mix):
+        assert a.error is None and b.error is None, (a.error, b.error)
+        for key in ("scalars", "radii", "perimeters", "extra_a", "sample_r"):
+            x, y = a.outputs[key], b.outputs[key]
+            assert np.array_equal(np.isnan(x), np.isnan(y))
+            fin = np.isfinite(x)
+            assert np.array_equal(x[fin], y[fin]), key  # same kernels, same order: bit-identical
