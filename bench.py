@@ -196,6 +196,9 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-sources", type=int, default=24)
     ap.add_argument("--profile-blocks", type=int, default=4)
+    ap.add_argument("--serving-requests", type=int, default=12,
+                    help="hemisphere requests of the seeded serving stream replayed at their arrival times (0 = skip)")
+    ap.add_argument("--serving-time-scale", type=float, default=1.0)
     ap.add_argument("--calibrate-bytes", type=int, default=0,
                     help="run one plain copy kernel of this many bytes first (byte-counter calibration)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
@@ -240,6 +243,20 @@ def main():
             raise
     best = max(results.values(), key=lambda r: r["sources_per_s"])
     dom, roofs = roofline(best, prep, best["info"]["block"])
+    serving = None
+    if args.serving_requests > 0:
+        from fastcorticalwiring_amd.serving import ServingEngine, stream_requests
+
+        log(f"bench: serving stream, {args.serving_requests} requests")
+        reqs = stream_requests(args.serving_requests)
+        eng = ServingEngine(max_active=3, block=args.block, precision=best["precision"], keep_outputs=False)
+        sres = eng.run(reqs, time_scale=args.serving_time_scale)
+        serving = ServingEngine.summary(sres)
+        serving.update(time_scale=args.serving_time_scale,
+                       levels=[r.meta["level"] for r in reqs],
+                       stream="folded icospheres L5/L6/L7 (30/40/30%), caps 5-20%, Poisson arrivals mean gap 5 s")
+        log(f"bench: serving p50 {serving['p50_s']:.2f}s p95 {serving['p95_s']:.2f}s "
+            f"p99 {serving['p99_s']:.2f}s, {serving['hemispheres_per_hour']:.0f} hemispheres/hour")
     cpu = None
     if cpu_proc is not None:
         log("bench: waiting for the CPU baseline leg")
@@ -280,6 +297,7 @@ def main():
                       "hemisphere_latency_s": v["hemisphere_latency_s"], "flagged": v["flagged"]}
                   for k, v in results.items()},
         "hemisphere_latency_s": best["hemisphere_latency_s"],
+        "serving": serving,
         "other_kernels": {k: {kk: v[kk] for kk in ("achieved", "unit", "frac", "avg_launch_ms", "launches")}
                           for k, v in roofs.items() if k != dom},
         "stage_s_per_block": {k: v / best["profile_blocks"] for k, v in best["stage_s"].items() if k != "wall"},

@@ -34,7 +34,8 @@ class Options(ctypes.Structure):
         ("device", ctypes.c_int),
         ("ir_steps", ctypes.c_int),
         ("streams", ctypes.c_int),
-        ("reserved", ctypes.c_int * 5),
+        ("sparse_rhs", ctypes.c_int),
+        ("reserved", ctypes.c_int * 4),
     ]
 
 

@@ -475,6 +475,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->device = 0;
     opt->ir_steps = 2;
     opt->streams = 2;
+    opt->sparse_rhs = 1;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -649,7 +650,7 @@ int fcw_distance_block(fcw_ctx* C, const int* src, int k, double* out) {
         Workspace& w = *C->ws[0];
         HIPCHK(hipMemcpy(w.src_pos.p, pos.data(), sizeof(int) * k, hipMemcpyHostToDevice));
         std::vector<BlockPlan> plans = C->make_plans(pos);
-        C->distances(w, k, &plans[0]);
+        C->distances(w, k, C->opt.sparse_rhs ? &plans[0] : nullptr);
         if (!C->Dout.p) C->Dout.alloc((size_t)C->n * C->ldk, C->dev_bytes);
         fcw::launch_gather_rows(w.stream, C->n, k, w.Dt.p, C->iperm.p, C->Dout.p);
         HIPCHK(hipGetLastError());
@@ -720,7 +721,7 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
             Workspace& w = *C->ws[bi % NS];
             const int k = std::min(C->ldk, nsrc - b0);
             HIPCHK(hipMemcpyAsync(w.src_pos.p, all_pos.p + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, w.stream));
-            C->distances(w, k, &plans[bi]);
+            C->distances(w, k, C->opt.sparse_rhs ? &plans[bi] : nullptr);
             EventPair* e = C->ev_begin(w, kMetrics);
             hipEvent_t ma = nullptr, mb = nullptr;
             if (C->profiling) {

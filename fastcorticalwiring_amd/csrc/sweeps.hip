@@ -8,6 +8,10 @@
 // Workgroup tile 64 x 64 (4 waves in 2 x 2, each wave 32 x 32 = 2 x 2 MFMA
 // blocks), depth 16 per step staged through LDS, next step prefetched into
 // registers while the current one runs on the matrix cores.
+//
+// Operand layout of v_mfma_f64_16x16x4f64 (one wave): A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, and the 16 x 16 result D[i][j] in lane j + 16 (i mod 4), register i / 4 (checked on
+// the device with a one-wave probe).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -130,7 +134,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
             const int col = n0 + wn0 + 16 * j + lrow;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = t.m0 + wm0 + 16 * i + 4 * lk + r;
+                // v_mfma_f64_16x16x4f64 result: lane l, register r holds D[(l >> 4) + 4 r][l & 15]
+                const int row = t.m0 + wm0 + 16 * i + lk + 4 * r;
                 if (row >= M) continue;
                 const double v = acc[i][j][r];
                 if (MODE == Sweep::Forward) {

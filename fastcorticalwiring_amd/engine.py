@@ -23,7 +23,7 @@ class HeatContext:
     """
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
-                 precision="fp64", ir_steps=2, device=0, host_threads=0):
+                 precision="fp64", ir_steps=2, device=0, host_threads=0, streams=2, sparse_rhs=True):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
         if V.ndim != 2 or V.shape[1] != 3:
@@ -42,6 +42,8 @@ class HeatContext:
         opt.host_threads = int(host_threads)
         opt.device = int(device)
         opt.ir_steps = int(ir_steps)
+        opt.streams = int(streams)
+        opt.sparse_rhs = 1 if sparse_rhs else 0
         handle = ctypes.c_void_p()
         _lib.check(L.fcw_create(_lib.dptr(V), V.shape[0], _lib.iptr(F), F.shape[0], ctypes.byref(opt),
                                 ctypes.byref(handle)))

@@ -51,7 +51,8 @@ typedef struct {
     int device;            /* HIP device ordinal */
     int ir_steps;          /* refinement sweeps per system in FCW_FP32_IR mode; default 2 */
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 2 */
-    int reserved[5];
+    int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
+    int reserved[4];
 } fcw_options;
 
 typedef struct {

@@ -91,10 +91,23 @@ def test_c2_distances_ragged_blocks(c2):
 
 
 def test_block_composition_does_not_change_results(c2):
+    """Distances are bit-identical whatever else shares the RHS block (no atomics in the solve)."""
     s, mesh, ctx, H = c2
     a = ctx.distance_block([5, 17, 300])
     b = ctx.distance_block([300] + list(range(1000, 1100)) + [17, 5])
     assert np.array_equal(a[0], b[-1]) and np.array_equal(a[1], b[-2]) and np.array_equal(a[2], b[0])
+
+
+def test_streams_and_sparse_rhs_give_identical_distances(c1):
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    s, mesh, ctx, H = c1
+    src = np.arange(0, mesh.n, 5)[:300]
+    ref = ctx.distances(src)
+    for streams, sparse in ((1, False), (1, True), (3, True)):
+        with HeatContext(mesh.V, mesh.F, block=128, streams=streams, sparse_rhs=sparse) as c:
+            D = c.distances(src)
+        assert np.max(np.abs(D - ref)) <= 1e-12 * np.max(ref), (streams, sparse)
 
 
 def test_geometry_fields_euclid_and_non_lipschitz(c2):
