@@ -82,14 +82,16 @@ enum TimedKernel { kFwdGemm = 0, kBwdGemm = 1, kMetrics = 2, kNumTimed = 3 };
 struct BlockPlan {
     std::vector<int> tile_off;  // per level, into the call's tile array
     std::vector<int> node_off;  // per level, into the call's node array
-    size_t flag_off = 0;        // into the call's flag array (nnodes bytes per block)
 };
 
 }  // namespace
 
+// Per-stream RHS-block buffers. The heat pair (Yh, Uh) is all zero between calls: the sparse
+// heat forward sweep writes only its active supernodes' rows and clears them afterwards, so
+// inactive children always contribute zero. The Poisson pair (Yp, Up) is rewritten densely.
 struct Workspace {
     hipStream_t stream = nullptr;
-    DBuf<double> X, Y, Z, U, Dt, band;
+    DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
     DBuf<int> src_pos;
     std::vector<EventPair> ev_pool;
     std::vector<std::pair<int, EventPair>> ev_pending;
@@ -109,7 +111,7 @@ struct fcw_ctx {
     DBuf<double> V, area, gb, larea, g11, g12, g22, lval, mass;
     DBuf<int> faces, vf_ptr, vf, rowptr, colind, cptr, contrib, iperm;
     // tree
-    DBuf<int> node_start, node_s, node_b, bidx, cmap, child_ptr, child_list, level_nodes;
+    DBuf<int> node_start, node_s, node_b, bidx, cmap, child_ptr, child_list, level_nodes, cown, ctail;
     DBuf<int64_t> node_boff, node_poff, node_uoff;
     DBuf<fcw::LevelTile> fwd_tiles, bwd_tiles;
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
@@ -118,7 +120,6 @@ struct fcw_ctx {
     // per-call plans
     DBuf<fcw::LevelTile> plan_tiles;
     DBuf<int> plan_nodes;
-    DBuf<unsigned char> plan_flags;
     // metrics
     bool metrics_ready = false;
     int S = 0, q = 0, nb = 0, band_cap = 6144;
@@ -195,52 +196,49 @@ struct fcw_ctx {
     void alloc_workspace(Workspace& w) {
         const size_t ld = (size_t)ldk;
         HIPCHK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+        const size_t urows = std::max<size_t>(1, (size_t)sym.urows);
         w.X.alloc((size_t)n * ld, dev_bytes);
-        w.Y.alloc((size_t)n * ld, dev_bytes);
+        w.Yh.alloc((size_t)n * ld, dev_bytes);
+        w.Yp.alloc((size_t)n * ld, dev_bytes);
         w.Z.alloc((size_t)n * ld, dev_bytes);
-        w.U.alloc(std::max<size_t>(1, (size_t)sym.urows * ld), dev_bytes);
+        w.Uh.alloc(urows * ld, dev_bytes);
+        w.Up.alloc(urows * ld, dev_bytes);
         w.Dt.alloc((size_t)n * ld, dev_bytes);
         w.src_pos.alloc(ld, dev_bytes);
+        fcw::launch_fill(w.stream, w.Yh.p, (int64_t)n * ldk, 0.0);
+        fcw::launch_fill(w.stream, w.Uh.p, (int64_t)urows * ldk, 0.0);
+        HIPCHK(hipStreamSynchronize(w.stream));
     }
 
-    // One system: forward then backward sweep. X (RHS, overwritten) -> Z (solution), Y scratch.
-    // With a plan, the forward sweep visits only the plan's active supernodes; every other
-    // node's forward result is zero (Y is cleared first).
-    void solve(Workspace& w, const double* panel, const BlockPlan* plan) {
+    // One system: forward then backward sweep, X (RHS) -> Z (solution) through Y/U. With a plan the
+    // forward sweep visits only the plan's active supernodes (Y/U must then be the heat pair, zero on
+    // every other row) and their rows are cleared again after the backward sweep.
+    void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U) {
         const int L = sym.nlevels;
-        if (plan) fcw::launch_fill(w.stream, w.Y.p, (int64_t)n * ldk, 0.0);
-        const unsigned char* flags = plan ? plan_flags.p + plan->flag_off : nullptr;
         for (int l = 0; l < L; ++l) {
             const fcw::LevelTile* tiles;
             int ntiles;
-            const int* nodes;
-            int nnodes;
             if (plan) {
                 tiles = plan_tiles.p + plan->tile_off[l];
                 ntiles = plan->tile_off[l + 1] - plan->tile_off[l];
-                nodes = plan_nodes.p + plan->node_off[l];
-                nnodes = plan->node_off[l + 1] - plan->node_off[l];
             } else {
                 tiles = fwd_tiles.p + fwd_tile_off[l];
                 ntiles = fwd_tile_off[l + 1] - fwd_tile_off[l];
-                nodes = level_nodes.p + level_off[l];
-                nnodes = level_off[l + 1] - level_off[l];
             }
-            if (l > 0)
-                fcw::launch_fwd_assemble(w.stream, nodes, nnodes, ldk, node_start.p, node_s.p, node_b.p, node_boff.p,
-                                         node_uoff.p, child_ptr.p, child_list.p, cmap.p, flags, w.X.p, w.U.p);
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
-                                 node_uoff.p, panel, w.X.p, w.Y.p, w.U.p, l > 0);
+                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
             EventPair* e = ev_begin(w, kBwdGemm);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y.p,
-                                 w.Z.p);
+                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p);
             ev_end(w, e);
         }
+        if (plan)
+            fcw::launch_clear_rows(w.stream, plan_nodes.p + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
+                                   ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
     }
 
     // Distances for the k sources whose positions are in w.src_pos[0:k]; result in w.Dt (k x n).
@@ -251,11 +249,14 @@ struct fcw_ctx {
             HIPCHK(hipEventRecord(ev[0], w.stream));
         }
         fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
-        solve(w, panel_heat.p, plan);
+        if (plan)
+            solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p);
+        else
+            solve(w, panel_heat.p, nullptr, w.Yp.p, w.Up.p);
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
         fcw::launch_divergence(w.stream, mesh_dev(), ldk, w.Z.p, w.X.p);
         if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
-        solve(w, panel_pois.p, nullptr);
+        solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p);
         fcw::launch_shift_transpose(w.stream, n, ldk, k, w.Z.p, w.src_pos.p, w.Dt.p);
         if (profiling) {
             HIPCHK(hipEventRecord(ev[3], w.stream));
@@ -276,20 +277,18 @@ struct fcw_ctx {
         std::vector<BlockPlan> plans(nblocks);
         std::vector<fcw::LevelTile> tiles;
         std::vector<int> nodes;
-        std::vector<unsigned char> flags((size_t)nblocks * nn, 0);
+        std::vector<int> mark(nn, -1);
         std::vector<std::vector<int>> per_level(L);
         for (int b = 0; b < nblocks; ++b) {
-            unsigned char* fl = flags.data() + (size_t)b * nn;
             for (auto& v : per_level) v.clear();
             const int j1 = std::min((int)pos.size(), (b + 1) * ldk);
             for (int j = b * ldk; j < j1; ++j) {
-                for (int id = node_of_pos[pos[j]]; id >= 0 && !fl[id]; id = sym.node_parent[id]) {
-                    fl[id] = 1;
+                for (int id = node_of_pos[pos[j]]; id >= 0 && mark[id] != b; id = sym.node_parent[id]) {
+                    mark[id] = b;
                     per_level[sym.node_level[id]].push_back(id);
                 }
             }
             BlockPlan& P = plans[b];
-            P.flag_off = (size_t)b * nn;
             P.tile_off.push_back((int)tiles.size());
             P.node_off.push_back((int)nodes.size());
             for (int l = 0; l < L; ++l) {
@@ -305,7 +304,6 @@ struct fcw_ctx {
         }
         plan_tiles.upload(tiles.data(), std::max<size_t>(1, tiles.size()), dev_bytes);
         plan_nodes.upload(nodes.data(), std::max<size_t>(1, nodes.size()), dev_bytes);
-        plan_flags.upload(flags.data(), std::max<size_t>(1, flags.size()), dev_bytes);
         return plans;
     }
 
@@ -578,6 +576,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->child_ptr.upload(S.child_ptr.data(), S.child_ptr.size(), tb);
         C->child_list.upload(S.child_list.data(), std::max<size_t>(1, S.child_list.size()), tb);
         C->level_nodes.upload(S.level_nodes.data(), S.level_nodes.size(), tb);
+        C->cown.upload(S.contrib_own.data(), S.contrib_own.size(), tb);
+        C->ctail.upload(S.contrib_tail.data(), S.contrib_tail.size(), tb);
         C->node_boff.upload(S.node_boff.data(), S.node_boff.size(), tb);
         C->node_poff.upload(S.node_poff.data(), S.node_poff.size(), tb);
         C->node_uoff.upload(S.node_uoff.data(), S.node_uoff.size(), tb);

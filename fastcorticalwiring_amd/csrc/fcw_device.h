@@ -82,13 +82,12 @@ void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, con
 void launch_fill(hipStream_t st, double* p, int64_t count, double value);
 void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes);
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);
-void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
-                         const int* node_s, const int* node_b, const int64_t* node_boff, const int64_t* node_uoff,
-                         const int* child_ptr, const int* child_list, const int* cmap, const unsigned char* active,
-                         double* X, double* U);
+void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
+                       const int* node_s, const int* node_b, const int64_t* node_uoff, double* Y, double* U);
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
-                     const double* panel, const double* X, double* Y, double* U, bool accumulate);
+                     const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
+                     double* U);
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z);

@@ -33,6 +33,9 @@ struct Symbolic {
     std::vector<int> level_ptr, level_nodes;  // nodes grouped by level (leaves = level 0)
     int64_t panel_elems = 0;
     int64_t urows = 0;
+    // Children's update rows feeding each row, two slots per row (-1 = none; the separator tree is
+    // binary): contrib_own[2p + i] for position p, contrib_tail[2u + i] for update row u.
+    std::vector<int> contrib_own, contrib_tail;
 
     // Full symmetric CSR of the operator pattern in positions (sorted columns, with diagonal).
     std::vector<int> rowptr, colind, diag;

@@ -100,36 +100,19 @@ __global__ void delta_rhs_kernel(double* __restrict__ X, int ldk, const int* __r
 // Level-scheduled supernodal sweeps with partitioned-inverse panels
 // ---------------------------------------------------------------------------
 
-// Children's update rows are folded into the node's own RHS rows (subtract) and
-// into its update rows (add). Children run one after another, so no atomics.
-__global__ __launch_bounds__(256) void fwd_assemble_kernel(
-    const int* __restrict__ nodes, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
-    const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int64_t* __restrict__ node_uoff,
-    const int* __restrict__ child_ptr, const int* __restrict__ child_list, const int* __restrict__ cmap,
-    const unsigned char* __restrict__ active, double* __restrict__ X, double* __restrict__ U) {
+// Zero the forward-sweep rows (Y rows and update rows) of the listed supernodes.
+__global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__ nodes, int ldk,
+                                                         const int* __restrict__ node_start,
+                                                         const int* __restrict__ node_s, const int* __restrict__ node_b,
+                                                         const int64_t* __restrict__ node_uoff, double* __restrict__ Y,
+                                                         double* __restrict__ U) {
     const int id = nodes[blockIdx.x];
     const int col = blockIdx.y * 64 + (threadIdx.x & 63);
     const int rg = threadIdx.x >> 6;
     const int s = node_s[id], b = node_b[id], start = node_start[id];
+    for (int r = rg; r < s; r += 4) Y[(int64_t)(start + r) * ldk + col] = 0.0;
     double* Ut = U + node_uoff[id] * ldk;
     for (int r = rg; r < b; r += 4) Ut[(int64_t)r * ldk + col] = 0.0;
-    __syncthreads();
-    for (int ci = child_ptr[id]; ci < child_ptr[id + 1]; ++ci) {
-        const int c = child_list[ci];
-        if (active && !active[c]) continue;  // subtree without sources: its update rows are zero
-        const int bc = node_b[c];
-        const int* cm = cmap + node_boff[c];
-        const double* Uc = U + node_uoff[c] * ldk;
-        for (int r = rg; r < bc; r += 4) {
-            const int t = cm[r];
-            const double v = Uc[(int64_t)r * ldk + col];
-            if (t < s)
-                X[(int64_t)(start + t) * ldk + col] -= v;
-            else
-                Ut[(int64_t)(t - s) * ldk + col] += v;
-        }
-        __syncthreads();
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -226,13 +209,10 @@ void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_
     delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
 }
 
-void launch_fwd_assemble(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
-                         const int* node_s, const int* node_b, const int64_t* node_boff, const int64_t* node_uoff,
-                         const int* child_ptr, const int* child_list, const int* cmap, const unsigned char* active,
-                         double* X, double* U) {
+void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
+                       const int* node_s, const int* node_b, const int64_t* node_uoff, double* Y, double* U) {
     if (count <= 0) return;
-    fwd_assemble_kernel<<<dim3(count, ldk / 64), 256, 0, st>>>(nodes, ldk, node_start, node_s, node_b, node_boff,
-                                                                node_uoff, child_ptr, child_list, cmap, active, X, U);
+    clear_rows_kernel<<<dim3(count, ldk / 64), 256, 0, st>>>(nodes, ldk, node_start, node_s, node_b, node_uoff, Y, U);
 }
 
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {

@@ -1,7 +1,10 @@
 // Level-scheduled supernodal sweeps as dense panel products on the FP64
 // matrix cores of gfx950 (v_mfma_f64_16x16x4f64).
 //
-// Forward, node S with assembled RHS rows X_S (s x k):   [Y_S ; U_S] (+)= P_S X_S
+// Forward, node S with RHS rows X_S (s x k):              [Y_S ; U_S] = P_S F_S + [0 ; T_S]
+//   F_S = X_S minus the children's update rows that land on S's own rows,
+//   T_S = the sum of the children's update rows that land on S's boundary rows
+//   (both gathered through per-row contributor maps while the tiles load).
 // Backward, with the ancestors' solution rows Z_B (b x k):  Z_S = P_S^T [Y_S ; -Z_B]
 // where P_S = [L11^{-1} ; L21 L11^{-1}] is the node's (s+b) x s column-major panel.
 //
@@ -35,8 +38,9 @@ template <Sweep MODE>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
-    const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const double* __restrict__ panel,
-    const double* __restrict__ Bsrc, const double* Zsrc, double* Out, double* __restrict__ U, int accumulate) {
+    const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
+    const int* __restrict__ ctail, const double* __restrict__ panel, const double* __restrict__ Bsrc,
+    const double* Zsrc, double* Out, double* U) {
     const LevelTile t = tiles[blockIdx.x];
     const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
     const double* __restrict__ P = panel + node_poff[t.node];
@@ -74,7 +78,13 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
             const int l = k0 + kk;
             double v = 0.0;
             if (MODE == Sweep::Forward) {
-                if (l < s) v = Bsrc[(int64_t)(start + l) * ldk + n0 + c];
+                if (l < s) {
+                    const int pos = start + l;
+                    v = Bsrc[(int64_t)pos * ldk + n0 + c];
+                    const int ca = cown[2 * pos], cb = cown[2 * pos + 1];
+                    if (ca >= 0) v -= U[(int64_t)ca * ldk + n0 + c];
+                    if (cb >= 0) v -= U[(int64_t)cb * ldk + n0 + c];
+                }
             } else {
                 if (l < s)
                     v = Bsrc[(int64_t)(start + l) * ldk + n0 + c];
@@ -126,7 +136,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         }
     }
 
-    double* Ut = (MODE == Sweep::Forward) ? U + node_uoff[t.node] * ldk : nullptr;
+    const int64_t uoff = (MODE == Sweep::Forward) ? node_uoff[t.node] : 0;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -142,8 +152,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                     if (row < s) {
                         Out[(int64_t)(start + row) * ldk + col] = v;
                     } else {
-                        double* u = Ut + (int64_t)(row - s) * ldk + col;
-                        *u = accumulate ? (*u + v) : v;
+                        const int64_t ur = uoff + (row - s);
+                        const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
+                        double w = v;
+                        if (ta >= 0) w += U[(int64_t)ta * ldk + col];
+                        if (tb >= 0) w += U[(int64_t)tb * ldk + col];
+                        U[ur * ldk + col] = w;
                     }
                 } else {
                     Out[(int64_t)(start + row) * ldk + col] = v;
@@ -157,11 +171,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
-                     const double* panel, const double* X, double* Y, double* U, bool accumulate) {
+                     const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
+                     double* U) {
     if (ntiles <= 0) return;
     sweep_kernel<Sweep::Forward><<<dim3(ntiles, ldk / kTileN), kThreads, 0, st>>>(
-        tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, panel, X, nullptr, Y, U,
-        accumulate ? 1 : 0);
+        tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail,
+        panel, X, nullptr, Y, U);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -169,7 +184,8 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z) {
     if (ntiles <= 0) return;
     sweep_kernel<Sweep::Backward><<<dim3(ntiles, ldk / kTileN), kThreads, 0, st>>>(
-        tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, panel, Y, Z, Z, nullptr, 0);
+        tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, Z,
+        nullptr);
 }
 
 }  // namespace fcw

@@ -386,6 +386,26 @@ int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic
     }
     S.panel_elems = S.node_poff[nn];
     S.urows = S.node_uoff[nn];
+    S.contrib_own.assign(2 * (size_t)n, -1);
+    S.contrib_tail.assign(2 * (size_t)std::max<int64_t>(1, S.urows), -1);
+    for (int c = 0; c < nn; ++c) {
+        const int p = S.node_parent[c];
+        if (p < 0) continue;
+        for (int r = 0; r < S.node_b[c]; ++r) {
+            const int t = S.cmap[S.node_boff[c] + r];
+            const int src_row = (int)(S.node_uoff[c] + r);
+            int* slot = (t < S.node_s[p]) ? &S.contrib_own[2 * (size_t)(S.node_start[p] + t)]
+                                          : &S.contrib_tail[2 * (size_t)(S.node_uoff[p] + t - S.node_s[p])];
+            if (slot[0] < 0) {
+                slot[0] = src_row;
+            } else if (slot[1] < 0) {
+                slot[1] = src_row;
+            } else {
+                err = "more than two children feed one row";
+                return 1;
+            }
+        }
+    }
 
     // ---- faces in positions, reordered by their smallest position -----------
     {
