@@ -116,6 +116,35 @@ __device__ inline int bin_of(double v, double wb, int nbins) {
     return k;
 }
 
+// Faces are visited UNR per lane per step: all index loads, then all gathers, then the work, so a
+// wave keeps 3 UNR independent gathers in flight instead of one dependent chain per face.
+constexpr int UNR = 4;
+
+struct FaceBatch {
+    int f[UNR];
+    double d0[UNR], d1[UNR], d2[UNR];
+};
+
+__device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ d, int m, int base,
+                                  int lane, FaceBatch& fb) {
+    int i0[UNR], i1[UNR], i2[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        const int f = base + u * kWave + lane;
+        fb.f[u] = f;
+        const int fs = f < m ? f : 0;  // an in-bounds face; the caller ignores f >= m
+        i0[u] = faces[3 * fs];
+        i1[u] = faces[3 * fs + 1];
+        i2[u] = faces[3 * fs + 2];
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+        fb.d0[u] = d[i0[u]];
+        fb.d1[u] = d[i1[u]];
+        fb.d2[u] = d[i2[u]];
+    }
+}
+
 struct Red {
     double d[NW][4];
     int i[NW][4];
@@ -186,24 +215,38 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     double v4[4] = {0.0, 0.0, 0.0, 0.0};  // sum d, sum d*w, sum w, max finite d
     int i4[4] = {0, 0, 0, 0};             // d < -h, chord violations, valid, non-finite
     const double h = md.health_h;
-    for (int p = tid; p < n; p += NT) {
-        const double dv = d[p];
-        if (!isfinite(dv)) {
-            i4[3] += 1;
-            continue;
+    for (int p0 = tid; p0 < n; p0 += NT * UNR) {
+        double dv[UNR], vx[UNR], vy[UNR], vz[UNR], vw[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int p = p0 + u * NT;
+            const int ps = p < n ? p : 0;
+            dv[u] = d[ps];
+            vx[u] = mesh.V[3 * ps];
+            vy[u] = mesh.V[3 * ps + 1];
+            vz[u] = mesh.V[3 * ps + 2];
+            vw[u] = md.vw[ps];
         }
-        const double dx = mesh.V[3 * p] - xs, dy = mesh.V[3 * p + 1] - ys, dz = mesh.V[3 * p + 2] - zs;
-        const double chord = sqrt(dx * dx + dy * dy + dz * dz);
-        if (dv < -h) i4[0] += 1;
-        if (dv < 0.9 * chord - h) i4[1] += 1;
-        if (dv > md.eps) {
-            const double w = md.vw[p];
-            i4[2] += 1;
-            v4[0] += dv;
-            v4[1] += dv * w;
-            v4[2] += w;
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            if (p0 + u * NT >= n) continue;
+            const double x = dv[u];
+            if (!isfinite(x)) {
+                i4[3] += 1;
+                continue;
+            }
+            const double dx = vx[u] - xs, dy = vy[u] - ys, dz = vz[u] - zs;
+            const double chord = sqrt(dx * dx + dy * dy + dz * dz);
+            if (x < -h) i4[0] += 1;
+            if (x < 0.9 * chord - h) i4[1] += 1;
+            if (x > md.eps) {
+                i4[2] += 1;
+                v4[0] += x;
+                v4[1] += x * vw[u];
+                v4[2] += vw[u];
+            }
+            v4[3] = fmax(v4[3], x);
         }
-        v4[3] = fmax(v4[3], dv);
     }
     block_reduce(red, v4, i4, 3, -1);
     const double dglob = v4[3];
@@ -262,20 +305,22 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
     const double wb = s_wb;
     const int nbins = s_nbins;
-    for (int f0 = wid * kWave; f0 < m; f0 += NT) {
-        const int f = f0 + lane;
-        int kh = -1, kl = -1;
-        double A = 0.0;
-        if (f < m) {
-            const double d0 = d[faces[3 * f]], d1 = d[faces[3 * f + 1]], d2 = d[faces[3 * f + 2]];
-            if (isfinite(d0) && isfinite(d1) && isfinite(d2)) {
-                A = mesh.larea[f];
+    for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+        FaceBatch fb;
+        load_faces(faces, d, m, base, lane, fb);
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            int kh = -1, kl = -1;
+            double A = 0.0;
+            const double d0 = fb.d0[u], d1 = fb.d1[u], d2 = fb.d2[u];
+            if (fb.f[u] < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
+                A = mesh.larea[fb.f[u]];
                 kh = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
                 kl = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
             }
+            keyed_add(Hmax, kh, A);
+            keyed_add(Hmin, kl, A);
         }
-        keyed_add(Hmax, kh, A);
-        keyed_add(Hmin, kl, A);
     }
     __syncthreads();
     // exclusive prefix sums in place: H[k] = sum over bins < k, k = 0..nbins
@@ -352,13 +397,16 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
 
     // ---- phase 2: band faces per scale, pre-sorted, contiguous -----------------------------
-    for (int f0 = wid * kWave; f0 < m; f0 += NT) {
-        const int f = f0 + lane;
-        int sa = S, sb = -1;
-        double fa = 0, fb = 0, fc = 0, A = 0, gn = 0;
-        if (f < m) {
-            const double d0 = d[faces[3 * f]], d1 = d[faces[3 * f + 1]], d2 = d[faces[3 * f + 2]];
-            if (isfinite(d0) && isfinite(d1) && isfinite(d2)) {
+    for (int base0 = wid * kWave * UNR; base0 < m; base0 += NT * UNR) {
+        FaceBatch fbat;
+        load_faces(faces, d, m, base0, lane, fbat);
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int f = fbat.f[u];
+            int sa = S, sb = -1;
+            double fa = 0, fb = 0, fc = 0, A = 0, gn = 0;
+            const double d0 = fbat.d0[u], d1 = fbat.d1[u], d2 = fbat.d2[u];
+            if (f < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
                 fa = d0;
                 fb = d1;
                 fc = d2;
@@ -382,26 +430,26 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     gn = grad_norm(d0, d1, d2, mesh.g11[f], mesh.g12[f], mesh.g22[f]);
                 }
             }
-        }
-        const int smin = wave_min_i(sa <= sb ? sa : S);
-        const int smax = wave_max_i(sa <= sb ? sb : -1);
-        for (int s = smin; s <= smax; ++s) {
-            const bool pred = (sa <= s && s <= sb);
-            const uint64_t mask = __ballot(pred);
-            if (mask == 0ull) continue;
-            const int leader = __ffsll((long long)mask) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&cnt[s], __popcll(mask));
-            base = __shfl(base, leader, kWave);
-            if (pred) {
-                const int slot = base + __popcll(mask & lanes_below(lane));
-                if (slot < cap) {
-                    double* bs = band + (int64_t)s * 5 * cap;
-                    bs[slot] = fa;
-                    bs[cap + slot] = fb;
-                    bs[2 * cap + slot] = fc;
-                    bs[3 * cap + slot] = A;
-                    bs[4 * cap + slot] = gn;
+            const int smin = wave_min_i(sa <= sb ? sa : S);
+            const int smax = wave_max_i(sa <= sb ? sb : -1);
+            for (int s = smin; s <= smax; ++s) {
+                const bool pred = (sa <= s && s <= sb);
+                const uint64_t mask = __ballot(pred);
+                if (mask == 0ull) continue;
+                const int leader = __ffsll((long long)mask) - 1;
+                int slot0 = 0;
+                if (lane == leader) slot0 = atomicAdd(&cnt[s], __popcll(mask));
+                slot0 = __shfl(slot0, leader, kWave);
+                if (pred) {
+                    const int slot = slot0 + __popcll(mask & lanes_below(lane));
+                    if (slot < cap) {
+                        double* bs = band + (int64_t)s * 5 * cap;
+                        bs[slot] = fa;
+                        bs[cap + slot] = fb;
+                        bs[2 * cap + slot] = fc;
+                        bs[3 * cap + slot] = A;
+                        bs[4 * cap + slot] = gn;
+                    }
                 }
             }
         }
@@ -524,15 +572,15 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     for (int v = tid; v < nv; v += NT) racc[v] = (rk[v] >= nbins) ? total : Hmax[rk[v]];
     __syncthreads();
     if (nv > 0) {
-        for (int f0 = wid * kWave; f0 < m; f0 += NT) {
-            const int f = f0 + lane;
-            int va = nv, vb = -1;
-            double fa = 0, fb = 0, fc = 0, A = 0;
-            if (f < m) {
-                fa = d[faces[3 * f]];
-                fb = d[faces[3 * f + 1]];
-                fc = d[faces[3 * f + 2]];
-                if (isfinite(fa) && isfinite(fb) && isfinite(fc)) {
+        for (int base0 = wid * kWave * UNR; base0 < m; base0 += NT * UNR) {
+            FaceBatch fbat;
+            load_faces(faces, d, m, base0, lane, fbat);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int f = fbat.f[u];
+                int va = nv, vb = -1;
+                double fa = fbat.d0[u], fb = fbat.d1[u], fc = fbat.d2[u], A = 0;
+                if (f < m && isfinite(fa) && isfinite(fb) && isfinite(fc)) {
                     sort3(fa, fb, fc);
                     const int bh = bin_of(fc, wb, nbins);
                     int lo = 0, hi = nv;  // first v with vr > fa
@@ -550,14 +598,14 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     vb = lo - 1;
                     A = mesh.larea[f];
                 }
-            }
-            const int vmin = wave_min_i(va <= vb ? va : nv);
-            const int vmax = wave_max_i(va <= vb ? vb : -1);
-            for (int v = vmin; v <= vmax; ++v) {
-                double a = 0.0, da;
-                if (va <= v && v <= vb && rk[v] < nbins) cdf_sorted(fa, fb, fc, A, vr[v], a, da);
-                const double sum = wave_sum(a);
-                if (lane == 0 && sum != 0.0) atomicAdd(&racc[v], sum);
+                const int vmin = wave_min_i(va <= vb ? va : nv);
+                const int vmax = wave_max_i(va <= vb ? vb : -1);
+                for (int v = vmin; v <= vmax; ++v) {
+                    double a = 0.0, da;
+                    if (va <= v && v <= vb && rk[v] < nbins) cdf_sorted(fa, fb, fc, A, vr[v], a, da);
+                    const double sum = wave_sum(a);
+                    if (lane == 0 && sum != 0.0) atomicAdd(&racc[v], sum);
+                }
             }
         }
     }
