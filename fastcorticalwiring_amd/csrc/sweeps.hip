@@ -9,8 +9,10 @@
 // where P_S = [L11^{-1} ; L21 L11^{-1}] is the node's (s+b) x s column-major panel.
 //
 // Workgroup tile 64 x 64 (4 waves in 2 x 2, each wave 32 x 32 = 2 x 2 MFMA
-// blocks), depth 16 per step staged through LDS, next step prefetched into
-// registers while the current one runs on the matrix cores.
+// blocks), depth 32 per step staged through LDS, next step prefetched into
+// registers while the current one runs on the matrix cores. The node's
+// boundary-row indices (backward) or child-contributor maps (forward) are staged
+// in LDS once per workgroup, so no tile load waits on an index load.
 //
 // Operand layout of v_mfma_f64_16x16x4f64 (one wave): A[i][k] in lane i + 16k, B[k][j] in lane
 // j + 16k, and the 16 x 16 result D[i][j] in lane j + 16 (i mod 4), register i / 4 (checked on
@@ -29,8 +31,9 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kPad = 2;
 constexpr int kLoadsA = kTileM * kTileK / kThreads;  // 4
-constexpr int kLoadsB = kTileK * kTileN / kThreads;  // 4
-static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling assumes 64 x 64 x 16");
+constexpr int kLoadsB = kTileK * kTileN / kThreads;  // 8
+constexpr int kIdxCap = 1024;                         // rows whose indices are staged in LDS
+static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling assumes 64 x 64 x 32");
 
 enum class Sweep { Forward, Backward };
 
@@ -53,6 +56,18 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 
     __shared__ double As[kTileK][kTileM + kPad];
     __shared__ double Bs[kTileK][kTileN + kPad];
+    __shared__ int sidx[2 * kIdxCap];
+    // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
+    // contributor rows for each of the node's s own rows.
+    const bool idx_lds = (MODE == Sweep::Backward) ? (b <= kIdxCap) : (s <= kIdxCap);
+    if (idx_lds) {
+        if (MODE == Sweep::Backward) {
+            for (int i = tid; i < b; i += kThreads) sidx[i] = B[i];
+        } else {
+            for (int i = tid; i < 2 * s; i += kThreads) sidx[i] = cown[2 * start + i];
+        }
+    }
+    __syncthreads();
 
     double ra[kLoadsA], rb[kLoadsB];
     auto fetch = [&](int k0) {
@@ -81,7 +96,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 if (l < s) {
                     const int pos = start + l;
                     v = Bsrc[(int64_t)pos * ldk + n0 + c];
-                    const int ca = cown[2 * pos], cb = cown[2 * pos + 1];
+                    const int ca = idx_lds ? sidx[2 * l] : cown[2 * pos];
+                    const int cb = idx_lds ? sidx[2 * l + 1] : cown[2 * pos + 1];
                     if (ca >= 0) v -= U[(int64_t)ca * ldk + n0 + c];
                     if (cb >= 0) v -= U[(int64_t)cb * ldk + n0 + c];
                 }
@@ -89,7 +105,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 if (l < s)
                     v = Bsrc[(int64_t)(start + l) * ldk + n0 + c];
                 else if (l < f)
-                    v = -Zsrc[(int64_t)B[l - s] * ldk + n0 + c];
+                    v = -Zsrc[(int64_t)(idx_lds ? sidx[l - s] : B[l - s]) * ldk + n0 + c];
             }
             rb[i] = v;
         }
