@@ -103,11 +103,11 @@ def cpu_worker(workload, n_sources, out_path, seed=0):
 # --------------------------------------------------------------------------------------------
 # GPU measurement
 # --------------------------------------------------------------------------------------------
-def run_gpu(prep, precision, block, steps, warmup, profile_blocks):
+def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=2):
     from fastcorticalwiring_amd.engine import HeatContext
 
     t0 = time.perf_counter()
-    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision)
+    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision, streams=streams)
     ctx.set_metrics(prep["targets"], prep["fracs"], prep["boundary"], prep["vertex_areas"],
                     health_h=prep["health_h"], bin_width=prep["width"])
     setup = time.perf_counter() - t0
@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=3)
     ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--streams", type=int, default=2, help="RHS blocks in flight (one HIP stream each)")
     ap.add_argument("--precision", default="auto", help="fp64 | fp32_ir | auto (every form available)")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-sources", type=int, default=24)
@@ -232,7 +233,8 @@ def main():
     for form in forms:
         try:
             log(f"bench: timing {form}")
-            results[form] = run_gpu(prep, form, args.block, args.steps, args.warmup, args.profile_blocks)
+            results[form] = run_gpu(prep, form, args.block, args.steps, args.warmup, args.profile_blocks,
+                                    args.streams)
             r = results[form]
             log(f"bench: {form}: {r['sources_per_s']:.1f} sources/s ({r['ms_per_step']:.2f} ms/block), "
                 f"setup {r['setup_s']:.2f}s")
@@ -285,6 +287,7 @@ def main():
             "surface_vertices": int(prep["n_full"]),
             "faces": int(prep["F"].shape[0]),
             "block": int(best["info"]["block"]),
+            "streams": int(args.streams),
             "precision": best["precision"],
             "scales": 12,
             "samples_between_scales": 3,

@@ -115,6 +115,7 @@ struct fcw_ctx {
     DBuf<int64_t> node_boff, node_poff, node_uoff;
     DBuf<fcw::LevelTile> fwd_tiles, bwd_tiles;
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
+    bool fwd_idx_lds = true, bwd_idx_lds = true;  // every node's sweep indices fit in LDS
     DBuf<double> panel_heat, panel_pois;
     DBuf<double> Dout;
     // per-call plans
@@ -227,13 +228,14 @@ struct fcw_ctx {
             }
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
-                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U);
+                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
             EventPair* e = ev_begin(w, kBwdGemm);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p);
+                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p,
+                                 bwd_idx_lds);
             ev_end(w, e);
         }
         if (plan)
@@ -597,6 +599,9 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         }
         C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
         C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
+        const int max_s = S.nnodes ? *std::max_element(S.node_s.begin(), S.node_s.end()) : 0;
+        C->fwd_idx_lds = max_s <= fcw::kSweepIdxCap;
+        C->bwd_idx_lds = S.max_front <= 2 * fcw::kSweepIdxCap;
         double t4 = now_s();
         fcw_info& I = C->info;
         I.n = n;

@@ -17,6 +17,10 @@ constexpr int kTileM = 64;
 constexpr int kTileN = 64;
 constexpr int kTileK = 16;
 constexpr int kColPad = 64;  // RHS block widths are padded to this many columns
+// Per-node row indices staged in LDS by the sweeps: the forward sweep needs s <= kSweepIdxCap
+// (two contributor rows per own row), the backward sweep s + b <= 2 * kSweepIdxCap (one source
+// row per depth index).
+constexpr int kSweepIdxCap = 1024;
 
 // Metric kernel limits.
 constexpr int kMaxScales = 32;
@@ -87,10 +91,10 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U);
+                     double* U, bool idx_lds);
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z);
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds);
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out);
 void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
                             double* Dt);

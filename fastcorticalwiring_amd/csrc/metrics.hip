@@ -125,12 +125,14 @@ struct FaceBatch {
     double d0[UNR], d1[UNR], d2[UNR];
 };
 
+// CONSEC = false: lane takes faces base + u*64 + lane; true: faces base + UNR*lane + u.
+template <bool CONSEC = false>
 __device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ d, int m, int base,
                                   int lane, FaceBatch& fb) {
     int i0[UNR], i1[UNR], i2[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        const int f = base + u * kWave + lane;
+        const int f = CONSEC ? base + UNR * lane + u : base + u * kWave + lane;
         fb.f[u] = f;
         const int fs = f < m ? f : 0;  // an in-bounds face; the caller ignores f >= m
         i0[u] = faces[3 * fs];
@@ -307,19 +309,44 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const int nbins = s_nbins;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         FaceBatch fb;
-        load_faces(faces, d, m, base, lane, fb);
+        load_faces<true>(faces, d, m, base, lane, fb);
+        // The lane's UNR faces are consecutive, so mostly share bins: merge equal neighbours
+        // first and hand the wave only the lane's distinct (bin, area) runs.
+        int kh[UNR], kl[UNR];
+        double A[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            int kh = -1, kl = -1;
-            double A = 0.0;
+            kh[u] = -1;
+            kl[u] = -1;
+            A[u] = 0.0;
             const double d0 = fb.d0[u], d1 = fb.d1[u], d2 = fb.d2[u];
             if (fb.f[u] < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
-                A = mesh.larea[fb.f[u]];
-                kh = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
-                kl = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
+                A[u] = mesh.larea[fb.f[u]];
+                kh[u] = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
+                kl[u] = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
             }
-            keyed_add(Hmax, kh, A);
-            keyed_add(Hmin, kl, A);
+        }
+#pragma unroll
+        for (int which = 0; which < 2; ++which) {
+            int* key = which == 0 ? kh : kl;
+            double* H = which == 0 ? Hmax : Hmin;
+            // runs of equal keys within the lane: run r starts at u = first index of the run
+            double run_sum[UNR];
+            int run_key[UNR];
+            int nrun = 0;
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                if (key[u] < 0) continue;
+                if (nrun > 0 && run_key[nrun - 1] == key[u]) {
+                    run_sum[nrun - 1] += A[u];
+                } else {
+                    run_key[nrun] = key[u];
+                    run_sum[nrun] = A[u];
+                    ++nrun;
+                }
+            }
+            const int rounds = wave_max_i(nrun);
+            for (int r = 0; r < rounds; ++r) keyed_add(H, r < nrun ? run_key[r] : -1, r < nrun ? run_sum[r] : 0.0);
         }
     }
     __syncthreads();
@@ -430,6 +457,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     gn = grad_norm(d0, d1, d2, mesh.g11[f], mesh.g12[f], mesh.g22[f]);
                 }
             }
+            if (__ballot(sa <= sb) == 0ull) continue;  // no band face in this wave
             const int smin = wave_min_i(sa <= sb ? sa : S);
             const int smax = wave_max_i(sa <= sb ? sb : -1);
             for (int s = smin; s <= smax; ++s) {
@@ -598,6 +626,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     vb = lo - 1;
                     A = mesh.larea[f];
                 }
+                if (__ballot(va <= vb) == 0ull) continue;  // no face straddles a sample radius
                 const int vmin = wave_min_i(va <= vb ? va : nv);
                 const int vmax = wave_max_i(va <= vb ? vb : -1);
                 for (int v = vmin; v <= vmax; ++v) {

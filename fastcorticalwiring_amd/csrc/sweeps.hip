@@ -9,10 +9,11 @@
 // where P_S = [L11^{-1} ; L21 L11^{-1}] is the node's (s+b) x s column-major panel.
 //
 // Workgroup tile 64 x 64 (4 waves in 2 x 2, each wave 32 x 32 = 2 x 2 MFMA
-// blocks), depth 32 per step staged through LDS, next step prefetched into
+// blocks), depth 16 per step staged through LDS, next step prefetched into
 // registers while the current one runs on the matrix cores. The node's
 // boundary-row indices (backward) or child-contributor maps (forward) are staged
-// in LDS once per workgroup, so no tile load waits on an index load.
+// in LDS once per workgroup (when they fit: kSweepIdxCap), so no tile load waits
+// on an index load.
 //
 // Operand layout of v_mfma_f64_16x16x4f64 (one wave): A[i][k] in lane i + 16k, B[k][j] in lane
 // j + 16k, and the 16 x 16 result D[i][j] in lane j + 16 (i mod 4), register i / 4 (checked on
@@ -31,13 +32,13 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kPad = 2;
 constexpr int kLoadsA = kTileM * kTileK / kThreads;  // 4
-constexpr int kLoadsB = kTileK * kTileN / kThreads;  // 8
-constexpr int kIdxCap = 1024;                         // rows whose indices are staged in LDS
-static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling assumes 64 x 64 x 32");
+constexpr int kLoadsB = kTileK * kTileN / kThreads;  // 4
+constexpr int kIdxCap = kSweepIdxCap;
+static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling assumes 64 x 64 x 16");
 
 enum class Sweep { Forward, Backward };
 
-template <Sweep MODE>
+template <Sweep MODE, bool IDX_LDS>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
@@ -59,70 +60,93 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     __shared__ int sidx[2 * kIdxCap];
     // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
     // contributor rows for each of the node's s own rows.
-    const bool idx_lds = (MODE == Sweep::Backward) ? (b <= kIdxCap) : (s <= kIdxCap);
-    if (idx_lds) {
+    if (IDX_LDS) {
         if (MODE == Sweep::Backward) {
-            for (int i = tid; i < b; i += kThreads) sidx[i] = B[i];
+            // source row of every depth index: own rows as positions, boundary rows as -(pos + 1)
+            for (int i = tid; i < f; i += kThreads) sidx[i] = i < s ? start + i : -(B[i - s] + 1);
         } else {
             for (int i = tid; i < 2 * s; i += kThreads) sidx[i] = cown[2 * start + i];
         }
     }
     __syncthreads();
 
-    double ra[kLoadsA], rb[kLoadsB];
+    // Every tile load below is unconditional: out-of-range elements read a clamped in-bounds
+    // address. fetch() only issues the loads of the next step (and records which elements are
+    // kept, subtracted or negated); stash() applies that after the current step's MFMAs, so no
+    // arithmetic on a loaded value forces a wait before the matrix cores have their work.
+    double ra[kLoadsA], rb[kLoadsB], ru[kLoadsB], rv[kLoadsB];
+    unsigned keep_a = 0, keep_b = 0, bit_u = 0, bit_v = 0;
     auto fetch = [&](int k0) {
+        keep_a = keep_b = bit_u = bit_v = 0;
 #pragma unroll
         for (int i = 0; i < kLoadsA; ++i) {
             const int e = tid + i * kThreads;
-            double v = 0.0;
+            int prow, pcol;  // element (prow, pcol) of the (s+b) x s panel
             if (MODE == Sweep::Forward) {
-                const int r = e % kTileM, kk = e / kTileM;  // P rows contiguous
-                const int row = t.m0 + r, col = k0 + kk;
-                if (row < f && col < s) v = P[row + (int64_t)col * f];
+                prow = t.m0 + e % kTileM;  // P rows contiguous
+                pcol = k0 + e / kTileM;
             } else {
-                const int kk = e % kTileK, r = e / kTileK;  // P^T: walk a panel column
-                const int orow = t.m0 + r, l = k0 + kk;
-                if (orow < s && l < f) v = P[l + (int64_t)orow * f];
+                prow = k0 + e % kTileK;  // P^T: walk a panel column
+                pcol = t.m0 + e / kTileK;
             }
-            ra[i] = v;
+            keep_a |= (prow < f && pcol < s) ? (1u << i) : 0u;
+            ra[i] = P[min(prow, f - 1) + (int64_t)min(pcol, s - 1) * f];
         }
 #pragma unroll
         for (int i = 0; i < kLoadsB; ++i) {
             const int e = tid + i * kThreads;
-            const int c = e % kTileN, kk = e / kTileN;
-            const int l = k0 + kk;
-            double v = 0.0;
+            const int c = n0 + e % kTileN, l = k0 + e / kTileN;
             if (MODE == Sweep::Forward) {
-                if (l < s) {
-                    const int pos = start + l;
-                    v = Bsrc[(int64_t)pos * ldk + n0 + c];
-                    const int ca = idx_lds ? sidx[2 * l] : cown[2 * pos];
-                    const int cb = idx_lds ? sidx[2 * l + 1] : cown[2 * pos + 1];
-                    if (ca >= 0) v -= U[(int64_t)ca * ldk + n0 + c];
-                    if (cb >= 0) v -= U[(int64_t)cb * ldk + n0 + c];
-                }
+                // F rows: own RHS row minus up to two children's update rows
+                const int lc = min(l, s - 1);
+                const int pos = start + lc;
+                const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * pos];
+                const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * pos + 1];
+                const double* xr = Bsrc + (int64_t)pos * ldk;
+                keep_b |= (l < s) ? (1u << i) : 0u;
+                bit_u |= (ca >= 0) ? (1u << i) : 0u;
+                bit_v |= (cb >= 0) ? (1u << i) : 0u;
+                rb[i] = xr[c];
+                ru[i] = U[(int64_t)max(ca, 0) * ldk + c];  // update row 0 stands in when unused
+                rv[i] = U[(int64_t)max(cb, 0) * ldk + c];
             } else {
-                if (l < s)
-                    v = Bsrc[(int64_t)(start + l) * ldk + n0 + c];
-                else if (l < f)
-                    v = -Zsrc[(int64_t)(idx_lds ? sidx[l - s] : B[l - s]) * ldk + n0 + c];
+                // [Y_S ; -Z_B] rows: own rows from Y, boundary rows negated from Z
+                const int lc = min(l, f - 1);
+                int code;  // position p of an own row, -(p + 1) of a boundary row
+                if (IDX_LDS)
+                    code = sidx[lc];
+                else
+                    code = lc < s ? start + lc : -(B[lc - s] + 1);
+                const bool bnd = code < 0;
+                const int prow = bnd ? -code - 1 : code;
+                keep_b |= (l < f) ? (1u << i) : 0u;
+                bit_u |= bnd ? (1u << i) : 0u;
+                // one load from either buffer, its base chosen by integer select (a pointer select
+                // is split by the compiler into two loads under branches)
+                const uintptr_t base = bnd ? reinterpret_cast<uintptr_t>(Zsrc) : reinterpret_cast<uintptr_t>(Bsrc);
+                rb[i] = reinterpret_cast<const double*>(base)[(int64_t)prow * ldk + c];
             }
-            rb[i] = v;
         }
     };
     auto stash = [&]() {
 #pragma unroll
         for (int i = 0; i < kLoadsA; ++i) {
             const int e = tid + i * kThreads;
+            const double v = ((keep_a >> i) & 1u) ? ra[i] : 0.0;
             if (MODE == Sweep::Forward)
-                As[e / kTileM][e % kTileM] = ra[i];
+                As[e / kTileM][e % kTileM] = v;
             else
-                As[e % kTileK][e / kTileK] = ra[i];
+                As[e % kTileK][e / kTileK] = v;
         }
 #pragma unroll
         for (int i = 0; i < kLoadsB; ++i) {
             const int e = tid + i * kThreads;
-            Bs[e / kTileN][e % kTileN] = rb[i];
+            double v;
+            if (MODE == Sweep::Forward)
+                v = rb[i] - (((bit_u >> i) & 1u) ? ru[i] : 0.0) - (((bit_v >> i) & 1u) ? rv[i] : 0.0);
+            else
+                v = ((bit_u >> i) & 1u) ? -rb[i] : rb[i];
+            Bs[e / kTileN][e % kTileN] = ((keep_b >> i) & 1u) ? v : 0.0;
         }
     };
 
@@ -170,10 +194,9 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                     } else {
                         const int64_t ur = uoff + (row - s);
                         const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
-                        double w = v;
-                        if (ta >= 0) w += U[(int64_t)ta * ldk + col];
-                        if (tb >= 0) w += U[(int64_t)tb * ldk + col];
-                        U[ur * ldk + col] = w;
+                        const double xa = U[(ta >= 0 ? (int64_t)ta : ur) * ldk + col];
+                        const double xb = U[(tb >= 0 ? (int64_t)tb : ur) * ldk + col];
+                        U[ur * ldk + col] = v + (ta >= 0 ? xa : 0.0) + (tb >= 0 ? xb : 0.0);
                     }
                 } else {
                     Out[(int64_t)(start + row) * ldk + col] = v;
@@ -188,20 +211,32 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U) {
+                     double* U, bool idx_lds) {
     if (ntiles <= 0) return;
-    sweep_kernel<Sweep::Forward><<<dim3(ntiles, ldk / kTileN), kThreads, 0, st>>>(
-        tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail,
-        panel, X, nullptr, Y, U);
+    const dim3 grid(ntiles, ldk / kTileN);
+    if (idx_lds)
+        sweep_kernel<Sweep::Forward, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, nullptr,
+                                                                      nullptr, node_poff, node_uoff, contrib_own,
+                                                                      contrib_tail, panel, X, nullptr, Y, U);
+    else
+        sweep_kernel<Sweep::Forward, false><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, nullptr,
+                                                                       nullptr, node_poff, node_uoff, contrib_own,
+                                                                       contrib_tail, panel, X, nullptr, Y, U);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z) {
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds) {
     if (ntiles <= 0) return;
-    sweep_kernel<Sweep::Backward><<<dim3(ntiles, ldk / kTileN), kThreads, 0, st>>>(
-        tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, Z,
-        nullptr);
+    const dim3 grid(ntiles, ldk / kTileN);
+    if (idx_lds)
+        sweep_kernel<Sweep::Backward, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
+                                                                       node_boff, bidx, node_poff, nullptr, nullptr,
+                                                                       nullptr, panel, Y, Z, Z, nullptr);
+    else
+        sweep_kernel<Sweep::Backward, false><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
+                                                                        node_boff, bidx, node_poff, nullptr, nullptr,
+                                                                        nullptr, panel, Y, Z, Z, nullptr);
 }
 
 }  // namespace fcw
