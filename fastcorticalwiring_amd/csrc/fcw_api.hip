@@ -123,7 +123,7 @@ struct fcw_ctx {
     DBuf<int> plan_nodes;
     // metrics
     bool metrics_ready = false;
-    int S = 0, q = 0, nb = 0, band_cap = 6144;
+    int S = 0, q = 0, nb = 0, band_cap = 6144;  // scratch: S * band_cap band faces per source
     double eps = 1e-6, health_h = 0.0, width = 0.0, cap = -1.0, rtol = 1e-10;
     int max_iter = 60;
     DBuf<double> targets, fracs, vw;
@@ -228,7 +228,7 @@ struct fcw_ctx {
             }
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
-                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds);
+                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
@@ -337,7 +337,7 @@ struct fcw_ctx {
         md.cap = cap;
         md.rtol = rtol;
         md.max_iter = max_iter;
-        md.band_cap = band_cap;
+        md.pool_cap = S * band_cap;
         md.band = w.band.p;
         return md;
     }

@@ -26,7 +26,7 @@ constexpr int kSweepIdxCap = 1024;
 constexpr int kMaxScales = 32;
 constexpr int kMaxExtras = 256;
 constexpr int kMaxBins = 2048;
-constexpr int kMetricThreads = 512;
+constexpr int kMetricThreads = 1024;
 
 // Per-source scalar outputs, in this order (double each).
 enum ScalarSlot {
@@ -65,8 +65,8 @@ struct MetricsDev {
     double eps, health_h, width, cap;  // cap < 0: no cap
     double rtol;
     int max_iter;
-    int band_cap;           // band faces per scale per source held in the scratch
-    double* band;           // k x S x 5 x band_cap (sorted a, b, c, area, |grad d| per band face)
+    int pool_cap;           // band faces per source held in the scratch (all bands together)
+    double* band;           // k x 5 x pool_cap (sorted a, b, c, area, |grad d| per band face)
 };
 
 struct MetricsOut {
@@ -91,7 +91,7 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds);
+                     double* U, bool idx_lds, bool leaves);
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds);

@@ -118,32 +118,103 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // ---------------------------------------------------------------------------
 // Fused gradient -> normalise -> divergence (fastcw_lean.py:416-442), gathered per vertex
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, const int* __restrict__ vf_ptr,
+//
+// A workgroup owns kDivRows consecutive vertex rows for one 64-column slice of the RHS block.
+// It first stages, cooperatively and in one pass, every incidence (face, corner) of its rows:
+// the face's vertex rows, the gradients of its basis functions and its area, in LDS. Each wave
+// then takes whole rows with one lane per column: a row's field values are loaded kDivC
+// incidences at a time with all loads in flight together, and every face quantity comes from
+// LDS (a broadcast read), so no global load waits on another. Each row sums its incidences in
+// CSR order, so the result does not depend on scheduling.
+//
+// The 1-D grid is laid out so that the column slices of one row group are dispatched to the
+// same XCD (workgroups are handed to the 8 XCDs round robin), where they share the staged
+// face data in that XCD's L2.
+constexpr int kDivRows = 16;   // rows per workgroup (4 per wave)
+constexpr int kDivInc = 256;   // incidences staged per pass (a pass per 256 when rows are long)
+constexpr int kDivC = 6;       // incidences in flight per row step (interior vertices have about 6)
+constexpr int kXcds = 8;
+
+__global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, const int* __restrict__ vf_ptr,
                                                          const int* __restrict__ vf, const int* __restrict__ faces,
                                                          const double* __restrict__ area,
                                                          const double* __restrict__ gb, const double* __restrict__ u,
                                                          double* __restrict__ out) {
-    const int col = blockIdx.y * 64 + (threadIdx.x & 63);
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    double acc = 0.0;
-    for (int e = vf_ptr[row]; e < vf_ptr[row + 1]; ++e) {
-        const int code = vf[e];
-        const int f = code / 3, a = code - 3 * f;
-        const int i0 = faces[3 * f], i1 = faces[3 * f + 1], i2 = faces[3 * f + 2];
-        const double u0 = u[(int64_t)i0 * ldk + col];
-        const double du1 = u[(int64_t)i1 * ldk + col] - u0;
-        const double du2 = u[(int64_t)i2 * ldk + col] - u0;
-        const double* g = gb + 9 * (int64_t)f;
-        const double gx = du1 * g[3] + du2 * g[6];
-        const double gy = du1 * g[4] + du2 * g[7];
-        const double gz = du1 * g[5] + du2 * g[8];
-        const double nn = sqrt(gx * gx + gy * gy + gz * gz);
-        if (!(nn > 1e-300)) continue;
-        const double sc = -area[f] / nn;
-        acc += sc * (gx * g[3 * a] + gy * g[3 * a + 1] + gz * g[3 * a + 2]);
+    __shared__ int4 rec[kDivInc];         // vertex rows i0, i1, i2 of the face, corner a
+    __shared__ double geo[kDivInc][10];   // grad phi_a (3), grad phi_1 (3), grad phi_2 (3), area
+    __shared__ int rptr[kDivRows + 1];
+
+    const int ncs = ldk / 64;
+    const int wg = blockIdx.x, xcd = wg % kXcds, k = wg / kXcds;
+    const int grp = (k / ncs) * kXcds + xcd, cs = k % ncs;
+    if (grp >= ngroups) return;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int col = cs * 64 + lane;
+    const int r0 = grp * kDivRows, r1 = min(n, r0 + kDivRows);
+    if (tid <= r1 - r0) rptr[tid] = vf_ptr[r0 + tid];
+    __syncthreads();
+    const int eb = rptr[0], ee = rptr[r1 - r0];
+
+    double acc[kDivRows / 4];
+#pragma unroll
+    for (int i = 0; i < kDivRows / 4; ++i) acc[i] = 0.0;
+
+    for (int base = eb; base < ee; base += kDivInc) {
+        const int cnt = min(kDivInc, ee - base);
+        if (base != eb) __syncthreads();  // the previous pass is done with the staged data
+        if (tid < cnt) {
+            const int code = vf[base + tid];
+            const int f = code / 3, a = code - 3 * f;
+            const int i0 = faces[3 * (int64_t)f], i1 = faces[3 * (int64_t)f + 1], i2 = faces[3 * (int64_t)f + 2];
+            const double* g = gb + 9 * (int64_t)f;
+            double gv[10];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gv[c] = g[3 * a + c];
+#pragma unroll
+            for (int c = 3; c < 9; ++c) gv[c] = g[c];
+            gv[9] = area[f];
+            rec[tid] = make_int4(i0, i1, i2, a);
+#pragma unroll
+            for (int c = 0; c < 10; ++c) geo[tid][c] = gv[c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kDivRows / 4; ++i) {
+            const int r = w + 4 * i;
+            if (r0 + r >= r1) break;
+            const int rs = max(rptr[r], base), re = min(rptr[r + 1], base + cnt);
+            for (int e = rs; e < re; e += kDivC) {
+                double u0[kDivC], u1[kDivC], u2[kDivC];
+#pragma unroll
+                for (int q = 0; q < kDivC; ++q) {
+                    const int4 R = rec[min(e + q, re - 1) - base];
+                    u0[q] = u[(int64_t)R.x * ldk + col];
+                    u1[q] = u[(int64_t)R.y * ldk + col];
+                    u2[q] = u[(int64_t)R.z * ldk + col];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // keep every load of the step ahead of the arithmetic
+#pragma unroll
+                for (int q = 0; q < kDivC; ++q) {
+                    // every step computes (a skip would pull its loads down to it), padding masked
+                    const double* G = geo[min(e + q, re - 1) - base];
+                    const double du1 = u1[q] - u0[q];
+                    const double du2 = u2[q] - u0[q];
+                    const double gx = du1 * G[3] + du2 * G[6];
+                    const double gy = du1 * G[4] + du2 * G[7];
+                    const double gz = du1 * G[5] + du2 * G[8];
+                    const double nn = sqrt(gx * gx + gy * gy + gz * gz);
+                    const double sc = -G[9] / nn;
+                    const double term = sc * (gx * G[0] + gy * G[1] + gz * G[2]);
+                    acc[i] += (e + q < re && nn > 1e-300) ? term : 0.0;
+                }
+            }
+        }
     }
-    out[(int64_t)row * ldk + col] = acc;
+#pragma unroll
+    for (int i = 0; i < kDivRows / 4; ++i) {
+        const int r = r0 + w + 4 * i;
+        if (r < r1) out[(int64_t)r * ldk + col] = acc[i];
+    }
 }
 
 // Dt[j][p] = phi[p][j] - phi[src_j][j]   (source-major distance fields)
@@ -216,8 +287,10 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 }
 
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {
-    divergence_kernel<<<dim3((mesh.n + 3) / 4, ldk / 64), 256, 0, st>>>(mesh.n, ldk, mesh.vf_ptr, mesh.vf,
-                                                                          mesh.faces, mesh.area, mesh.gb, u, out);
+    const int ngroups = (mesh.n + kDivRows - 1) / kDivRows;
+    const int padded = (ngroups + kXcds - 1) / kXcds * kXcds;  // whole rounds of the XCD mapping
+    divergence_kernel<<<dim3(padded * (ldk / 64)), 256, 0, st>>>(mesh.n, ldk, ngroups, mesh.vf_ptr, mesh.vf,
+                                                                 mesh.faces, mesh.area, mesh.gb, u, out);
 }
 
 void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,

@@ -3,21 +3,30 @@
 //
 //   phase 0  vertex pass: health screen, MSD (unweighted / area-weighted),
 //            distance to boundary, largest finite distance
-//   phase 1  face pass: area histograms of per-face dmax and dmin. For any r on
-//            the bin grid, sum(dmax < r) <= A(r) <= sum(dmin < r), which brackets
-//            every target radius to a few bins before any exact evaluation.
-//   phase 2  face pass: the faces that can change A(r) inside each bracket
-//            ("band") are copied, pre-sorted, into contiguous per-scale arrays
-//   phase 3  safeguarded Newton on the exact piecewise-quadratic A(r) (one wave
+//   phase 1  face pass: per-bin face area and face count of dmax and of dmin.
+//            For r on the bin grid, sum(dmax < r) <= A(r) <= sum(dmin < r), which
+//            brackets every target radius to a few bins before any exact
+//            evaluation; the counts bound how many faces a bracket can hold.
+//   phase 2  brackets: a band [kL wb, H] per target radius and one per
+//            supplementary sample radius (bracketed from the brackets of the two
+//            radii it lies between); each band gets an exact-size slot range in
+//            the per-source scratch.
+//   phase 3  face pass: every face whose [dmin, dmax] meets a band is copied,
+//            pre-sorted, into that band. For r in [kL wb, H],
+//            A(r) = (area of faces with dmax bin < kL) + sum over the band of the
+//            face's exact area inside r.
+//   phase 4  safeguarded Newton on the exact piecewise-quadratic A(r) (one wave
 //            per scale), perimeter by the coarea formula at the root
-//   phase 4  face pass: exact areas at the log-spaced samples between radii
-//   phase 5  outputs and the cleaned (radius, area) samples
+//   phase 5  exact areas at the supplementary sample radii, one band each
+//   phase 6  outputs and the cleaned (radius, area) samples
 //
-// Face passes are coalesced (consecutive lanes take consecutive faces); LDS
-// histogram and accumulator updates are summed per run of equal keys inside a
-// wave first, so one atomic is issued per run.
+// Face passes are coalesced and issue all loads of a step before any is used;
+// histogram updates are LDS atomics (lanes cover consecutive faces, so they
+// rarely collide). Sums whose order depends on scheduling (histogram bins,
+// band contents) only steer the work or change results in the last bits.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -29,6 +38,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int NT = kMetricThreads;
 constexpr int NW = NT / kWave;
+constexpr int kMaxBands = kMaxScales + kMaxExtras;
 
 __device__ inline int lane_id() { return threadIdx.x & (kWave - 1); }
 
@@ -58,25 +68,6 @@ __device__ inline int wave_max_i(int v) {
 }
 
 __device__ inline uint64_t lanes_below(int lane) { return (1ull << lane) - 1ull; }
-__device__ inline uint64_t lanes_upto(int lane) { return lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull); }
-
-// H[key] += v for every lane with key >= 0. Runs of equal consecutive keys are summed by a
-// segmented scan and the last lane of each run issues the LDS atomic. All 64 lanes must call.
-__device__ inline void keyed_add(double* H, int key, double v) {
-    const int lane = lane_id();
-    const int prev = __shfl_up(key, 1, kWave);
-    const bool head = (lane == 0) || (prev != key);
-    const uint64_t hm = __ballot(head);
-    const int rid = __popcll(hm & lanes_upto(lane));
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const double vv = __shfl_up(v, o, kWave);
-        const int rr = __shfl_up(rid, o, kWave);
-        if (lane >= o && rr == rid) v += vv;
-    }
-    const bool tail = (lane == kWave - 1) || ((hm >> (lane + 1)) & 1ull);
-    if (tail && key >= 0) atomicAdd(&H[key], v);
-}
 
 __device__ inline void sort3(double& a, double& b, double& c) {
     double t;
@@ -116,28 +107,28 @@ __device__ inline int bin_of(double v, double wb, int nbins) {
     return k;
 }
 
-// Faces are visited UNR per lane per step: all index loads, then all gathers, then the work, so a
-// wave keeps 3 UNR independent gathers in flight instead of one dependent chain per face.
+// Faces are visited UNR per lane per step: all index loads, then all gathers (and the face
+// area), so a wave keeps several independent gathers in flight.
 constexpr int UNR = 4;
 
 struct FaceBatch {
     int f[UNR];
-    double d0[UNR], d1[UNR], d2[UNR];
+    double d0[UNR], d1[UNR], d2[UNR], A[UNR];
 };
 
-// CONSEC = false: lane takes faces base + u*64 + lane; true: faces base + UNR*lane + u.
-template <bool CONSEC = false>
-__device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ d, int m, int base,
-                                  int lane, FaceBatch& fb) {
+// Lane takes faces base + UNR*lane + u (consecutive faces per lane, 256 consecutive per wave).
+__device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ larea,
+                                  const double* __restrict__ d, int m, int base, int lane, FaceBatch& fb) {
     int i0[UNR], i1[UNR], i2[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        const int f = CONSEC ? base + UNR * lane + u : base + u * kWave + lane;
+        const int f = base + UNR * lane + u;
         fb.f[u] = f;
-        const int fs = f < m ? f : 0;  // an in-bounds face; the caller ignores f >= m
+        const int fs = f < m ? f : m - 1;  // an in-bounds face; the caller ignores f >= m
         i0[u] = faces[3 * fs];
         i1[u] = faces[3 * fs + 1];
         i2[u] = faces[3 * fs + 2];
+        fb.A[u] = larea[fs];
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
@@ -145,6 +136,7 @@ __device__ inline void load_faces(const int* __restrict__ faces, const double* _
         fb.d1[u] = d[i1[u]];
         fb.d2[u] = d[i2[u]];
     }
+    __builtin_amdgcn_sched_barrier(0);  // every load of the step is issued before any use
 }
 
 struct Red {
@@ -183,6 +175,72 @@ __device__ void block_reduce(Red& sm, double (&v)[4], int (&iv)[4], int smax, in
     __syncthreads();
 }
 
+// Log-spaced supplementary radius between r0 < r1 (lean_geometry.py:320-330).
+__device__ inline double log_mix(double r0, double r1, double fr) { return exp(log(r0) + fr * (log(r1) - log(r0))); }
+
+// One band's sums at radius r over lanes of one wave: area inside r beyond the fully-inside
+// part, its derivative, and the coarea perimeter density. `pool` holds the band's sorted
+// (a, b, c), area and |grad d| in five arrays of stride `pstride`. With `full`, the band is
+// instead recomputed from every face (the band did not fit in the scratch).
+struct BandRef {
+    int kL;
+    double H;
+    int off, count;
+    bool full;
+};
+
+__device__ void band_sums(const BandRef& B, double r, const double* __restrict__ pool, int64_t pstride,
+                          const int* __restrict__ faces, const double* __restrict__ d,
+                          const double* __restrict__ larea, const double* __restrict__ g11,
+                          const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
+                          int nbins, double& sA, double& sdA, double& sP) {
+    const int lane = lane_id();
+    sA = sdA = sP = 0.0;
+    if (!B.full) {
+        constexpr int U = 4;
+        const double* pa = pool + B.off;
+        const double* pb = pa + pstride;
+        const double* pc = pb + pstride;
+        const double* pA = pc + pstride;
+        const double* pg = pA + pstride;
+        for (int i0 = lane; i0 < B.count; i0 += U * kWave) {
+            double a[U], b[U], c[U], A[U], g[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int i = min(i0 + u * kWave, B.count - 1);
+                a[u] = pa[i];
+                b[u] = pb[i];
+                c[u] = pc[i];
+                A[u] = pA[i];
+                g[u] = pg[i];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                double x, dx;
+                cdf_sorted(a[u], b[u], c[u], A[u], r, x, dx);
+                const bool ok = i0 + u * kWave < B.count;
+                sA += ok ? x : 0.0;
+                sdA += ok ? dx : 0.0;
+                sP += (ok && dx > 0.0) ? dx * g[u] : 0.0;
+            }
+        }
+    } else {
+        for (int f = lane; f < m; f += kWave) {
+            const double d0 = d[faces[3 * f]], d1 = d[faces[3 * f + 1]], d2 = d[faces[3 * f + 2]];
+            if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) continue;
+            double fa = d0, fb = d1, fc = d2;
+            sort3(fa, fb, fc);
+            if (!(bin_of(fc, wb, nbins) >= B.kL && fa < B.H)) continue;
+            double x, dx;
+            cdf_sorted(fa, fb, fc, larea[f], r, x, dx);
+            sA += x;
+            sdA += dx;
+            if (dx > 0.0) sP += dx * grad_norm(d0, d1, d2, g11[f], g12[f], g22[f]);
+        }
+    }
+}
+
 __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md, const double* __restrict__ Dt,
                                                       const int* __restrict__ src_pos, MetricsOut out,
                                                       int health_mode) {
@@ -192,23 +250,34 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const int* __restrict__ faces = mesh.faces;
     const double* __restrict__ d = Dt + (int64_t)j * n;
     const int S = md.S, q = md.q, NE = (S > 0 ? (S - 1) * q : 0);
+    const int NB = S + NE;  // bands: scale s at s (q + 1), its samples (s, jj) at s (q + 1) + 1 + jj
     double* scal = out.scal + (int64_t)j * kSlotCount;
     double* radii = out.radii + (int64_t)j * S;
     double* perim = out.perim + (int64_t)j * S;
     double* er = out.extra_r + (int64_t)j * NE;
     double* ea = out.extra_a + (int64_t)j * NE;
-    const int cap = md.band_cap;
-    double* band = md.band + (int64_t)j * S * 5 * cap;
+    const int64_t pstride = md.pool_cap;
+    double* pool = md.band + (int64_t)j * 5 * pstride;
 
     __shared__ Red red;
     __shared__ double Hmax[kMaxBins + 1];
     __shared__ double Hmin[kMaxBins + 1];
-    __shared__ int lo_k[kMaxScales], hi_k[kMaxScales], cnt[kMaxScales], act[kMaxScales], ev_out[kMaxScales];
+    // Face counts per bin (phases 1-2); afterwards the same storage holds the sample arrays.
+    __shared__ union {
+        struct {
+            unsigned cmax[kMaxBins + 1];
+            unsigned cmin[kMaxBins + 1];
+        } c;
+        struct {
+            double rho[kMaxExtras], acc[kMaxExtras];
+            double pair_r[kMaxBands], pair_a[kMaxBands];
+        } x;
+    } U;
+    __shared__ int b_kL[kMaxBands], b_cnt[kMaxBands], b_off[kMaxBands], b_cap[kMaxBands];
+    __shared__ double b_H[kMaxBands];
+    __shared__ int lo_k[kMaxScales], act[kMaxScales], ev_out[kMaxScales];
     __shared__ double hi_r[kMaxScales], r_out[kMaxScales], p_out[kMaxScales];
-    __shared__ double rho[kMaxExtras], vr[kMaxExtras], racc[kMaxExtras];
-    __shared__ int rk[kMaxExtras], ridx[kMaxExtras];
-    __shared__ double pair_r[kMaxScales + kMaxExtras], pair_a[kMaxScales + kMaxExtras];
-    __shared__ int s_nv, s_nbins;
+    __shared__ int s_nbins, s_ovf;
     __shared__ double s_wb;
 
     // ---- phase 0: health screen, MSD, distance to boundary -------------------------------
@@ -229,6 +298,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             vz[u] = mesh.V[3 * ps + 2];
             vw[u] = md.vw[ps];
         }
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             if (p0 + u * NT >= n) continue;
@@ -289,7 +359,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         return;
     }
 
-    // ---- phase 1: dmax / dmin area histograms ----------------------------------------------
+    // ---- phase 1: dmax / dmin area and count histograms ------------------------------------
     if (tid == 0) {
         // Bins only steer the work, never the result: a quarter of the reference bucket width,
         // widened when the field is too long for the LDS histogram.
@@ -299,79 +369,76 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         if (nbins > kMaxBins) nbins = kMaxBins;
         s_wb = wb;
         s_nbins = nbins;
+        s_ovf = 0;
     }
     for (int i = tid; i <= kMaxBins; i += NT) {
         Hmax[i] = 0.0;
         Hmin[i] = 0.0;
+        U.c.cmax[i] = 0u;
+        U.c.cmin[i] = 0u;
     }
     __syncthreads();
     const double wb = s_wb;
     const int nbins = s_nbins;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         FaceBatch fb;
-        load_faces<true>(faces, d, m, base, lane, fb);
-        // The lane's UNR faces are consecutive, so mostly share bins: merge equal neighbours
-        // first and hand the wave only the lane's distinct (bin, area) runs.
-        int kh[UNR], kl[UNR];
-        double A[UNR];
+        load_faces(faces, mesh.larea, d, m, base, lane, fb);
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            kh[u] = -1;
-            kl[u] = -1;
-            A[u] = 0.0;
             const double d0 = fb.d0[u], d1 = fb.d1[u], d2 = fb.d2[u];
             if (fb.f[u] < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
-                A[u] = mesh.larea[fb.f[u]];
-                kh[u] = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
-                kl[u] = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
+                const int kh = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
+                const int kl = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
+                atomicAdd(&Hmax[kh], fb.A[u]);
+                atomicAdd(&Hmin[kl], fb.A[u]);
+                atomicAdd(&U.c.cmax[kh], 1u);
+                atomicAdd(&U.c.cmin[kl], 1u);
             }
-        }
-#pragma unroll
-        for (int which = 0; which < 2; ++which) {
-            int* key = which == 0 ? kh : kl;
-            double* H = which == 0 ? Hmax : Hmin;
-            // runs of equal keys within the lane: run r starts at u = first index of the run
-            double run_sum[UNR];
-            int run_key[UNR];
-            int nrun = 0;
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                if (key[u] < 0) continue;
-                if (nrun > 0 && run_key[nrun - 1] == key[u]) {
-                    run_sum[nrun - 1] += A[u];
-                } else {
-                    run_key[nrun] = key[u];
-                    run_sum[nrun] = A[u];
-                    ++nrun;
-                }
-            }
-            const int rounds = wave_max_i(nrun);
-            for (int r = 0; r < rounds; ++r) keyed_add(H, r < nrun ? run_key[r] : -1, r < nrun ? run_sum[r] : 0.0);
         }
     }
     __syncthreads();
-    // exclusive prefix sums in place: H[k] = sum over bins < k, k = 0..nbins
-    if (wid < 2) {
-        double* H = (wid == 0) ? Hmax : Hmin;
+    // exclusive prefix sums in place: X[k] = sum over bins < k, k = 0..nbins
+    if (wid < 4) {
         const int per = (nbins + kWave - 1) / kWave;
         const int b0 = lane * per, b1 = min(nbins, b0 + per);
-        double loc = 0.0;
-        for (int b = b0; b < b1; ++b) loc += H[b];
-        double inc = loc;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const double y = __shfl_up(inc, o, kWave);
-            if (lane >= o) inc += y;
+        if (wid < 2) {
+            double* H = (wid == 0) ? Hmax : Hmin;
+            double loc = 0.0;
+            for (int b = b0; b < b1; ++b) loc += H[b];
+            double inc = loc;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const double y = __shfl_up(inc, o, kWave);
+                if (lane >= o) inc += y;
+            }
+            double run = inc - loc;
+            for (int b = b0; b < b1; ++b) {
+                const double hv = H[b];
+                H[b] = run;
+                run += hv;
+            }
+            if (lane == kWave - 1) H[nbins] = inc;
+        } else {
+            unsigned* C = (wid == 2) ? U.c.cmax : U.c.cmin;
+            unsigned loc = 0;
+            for (int b = b0; b < b1; ++b) loc += C[b];
+            unsigned inc = loc;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const unsigned y = __shfl_up(inc, o, kWave);
+                if (lane >= o) inc += y;
+            }
+            unsigned run = inc - loc;
+            for (int b = b0; b < b1; ++b) {
+                const unsigned cv = C[b];
+                C[b] = run;
+                run += cv;
+            }
+            if (lane == kWave - 1) C[nbins] = inc;
         }
-        double run = inc - loc;
-        for (int b = b0; b < b1; ++b) {
-            const double hv = H[b];
-            H[b] = run;
-            run += hv;
-        }
-        if (lane == kWave - 1) H[nbins] = inc;
     }
     __syncthreads();
     const double total = Hmax[nbins];
+
+    // ---- phase 2: brackets and bands ------------------------------------------------------
     if (tid < S) {
         const int s = tid;
         const double T = md.targets[s];
@@ -391,7 +458,6 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         r_out[s] = r0v;
         p_out[s] = NAN;
         ev_out[s] = 0;
-        cnt[s] = 0;
         if (a == 1) {
             int lo = 0, hi = nbins;  // first k with Hmax[k] >= T  (A(k wb) >= Hmax[k])
             if (Hmax[nbins] < T) {
@@ -409,91 +475,180 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                 if (Hmin[mid] < T) l2 = mid; else h2 = mid - 1;
             }
             lo_k[s] = l2;
-            hi_k[s] = khi;
             hi_r[s] = (double)khi * wb;
-        } else if (a == 0) {
-            lo_k[s] = 0;
-            hi_k[s] = 0;
-            hi_r[s] = -INFINITY;
         } else {
-            lo_k[s] = 0x7fffffff;
-            hi_k[s] = 0x7fffffff;
-            hi_r[s] = INFINITY;
+            lo_k[s] = 0;
+            hi_r[s] = 0.0;
         }
     }
     __syncthreads();
+    // Band brackets. An empty band has kL = INT_MAX (no dmax bin reaches it) and H = -inf
+    // before the first active scale or +inf after the last.
+    for (int i = tid; i < NB; i += NT) {
+        const int s = i / (q + 1), jj = i - s * (q + 1) - 1;
+        int kL = INT_MAX;
+        double H = (s < S && act[s] == 2) ? INFINITY : -INFINITY;
+        if (jj < 0) {
+            if (act[s] == 1) {
+                kL = lo_k[s];
+                H = hi_r[s];
+            }
+        } else if (act[s] == 1 && act[s + 1] == 1) {
+            // the sample radius is increasing in both radii it lies between; one bin of slack
+            // on each side covers the rounding of exp/log
+            const double fr = md.fracs[jj];
+            const double l0 = (double)lo_k[s] * wb, l1 = fmax((double)lo_k[s + 1] * wb, l0);
+            const double h0 = hi_r[s], h1 = fmax(hi_r[s + 1], h0);
+            const double rlo = (l0 > 0.0) ? log_mix(l0, l1, fr) : 0.0;
+            const double rhi = log_mix(h0, h1, fr);
+            kL = max(0, (int)floor(rlo / wb) - 1);
+            H = rhi + wb;
+        } else if (act[s + 1] == 2) {
+            H = INFINITY;
+        }
+        b_kL[i] = kL;
+        b_H[i] = H;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // Monotone in band order (a band may only widen): then each face's bands form one
+        // contiguous range, found by two binary searches.
+        double hm = -INFINITY;
+        for (int i = 0; i < NB; ++i) {
+            hm = fmax(hm, b_H[i]);
+            b_H[i] = hm;
+        }
+        int km = INT_MAX;
+        for (int i = NB - 1; i >= 0; --i) {
+            km = min(km, b_kL[i]);
+            b_kL[i] = km;
+        }
+        // capacity: faces with dmin < H minus faces with dmax bin < kL, exact-size offsets
+        int64_t off = 0;
+        int ovf = 0;
+        for (int i = 0; i < NB; ++i) {
+            int cap = 0;
+            if (b_kL[i] < nbins && b_H[i] > -INFINITY) {
+                const unsigned below_h = (b_H[i] == INFINITY) ? U.c.cmin[nbins]
+                                                              : U.c.cmin[bin_of(b_H[i], wb, nbins) + 1];
+                const unsigned inside = U.c.cmax[b_kL[i]];
+                cap = below_h > inside ? (int)(below_h - inside) : 0;
+            }
+            if (off + cap > md.pool_cap) {
+                b_off[i] = -1;  // recomputed from every face when evaluated
+                b_cap[i] = 0;
+                ovf = 1;
+            } else {
+                b_off[i] = (int)off;
+                b_cap[i] = cap;
+                off += cap;
+            }
+            b_cnt[i] = 0;
+        }
+        s_ovf = ovf;
+    }
+    __syncthreads();
 
-    // ---- phase 2: band faces per scale, pre-sorted, contiguous -----------------------------
-    for (int base0 = wid * kWave * UNR; base0 < m; base0 += NT * UNR) {
-        FaceBatch fbat;
-        load_faces(faces, d, m, base0, lane, fbat);
+    // ---- phase 3: band faces, pre-sorted, contiguous per band -----------------------------
+    for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+        FaceBatch fb;
+        load_faces(faces, mesh.larea, d, m, base, lane, fb);
+        int ia[UNR], ib[UNR];
+        double sa_[UNR], sb_[UNR], sc_[UNR];
+        bool anyf = false;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const int f = fbat.f[u];
-            int sa = S, sb = -1;
-            double fa = 0, fb = 0, fc = 0, A = 0, gn = 0;
-            const double d0 = fbat.d0[u], d1 = fbat.d1[u], d2 = fbat.d2[u];
-            if (f < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
-                fa = d0;
-                fb = d1;
-                fc = d2;
-                sort3(fa, fb, fc);
+            ia[u] = NB;
+            ib[u] = -1;
+            double fa = fb.d0[u], fbv = fb.d1[u], fc = fb.d2[u];
+            if (fb.f[u] < m && isfinite(fa) && isfinite(fbv) && isfinite(fc)) {
+                sort3(fa, fbv, fc);
                 const int bh = bin_of(fc, wb, nbins);
-                int lo = 0, hi = S;  // first s with hi_r[s] > fa
+                int lo = 0, hi = NB;  // first band with H > dmin
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    if (hi_r[mid] > fa) hi = mid; else lo = mid + 1;
+                    if (b_H[mid] > fa) hi = mid; else lo = mid + 1;
                 }
-                sa = lo;
+                ia[u] = lo;
                 lo = 0;
-                hi = S;  // first s with lo_k[s] > bh
+                hi = NB;  // first band with kL > dmax bin
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    if (lo_k[mid] > bh) hi = mid; else lo = mid + 1;
+                    if (b_kL[mid] > bh) hi = mid; else lo = mid + 1;
                 }
-                sb = lo - 1;
-                if (sa <= sb) {
-                    A = mesh.larea[f];
-                    gn = grad_norm(d0, d1, d2, mesh.g11[f], mesh.g12[f], mesh.g22[f]);
-                }
+                ib[u] = lo - 1;
             }
-            if (__ballot(sa <= sb) == 0ull) continue;  // no band face in this wave
-            const int smin = wave_min_i(sa <= sb ? sa : S);
-            const int smax = wave_max_i(sa <= sb ? sb : -1);
-            for (int s = smin; s <= smax; ++s) {
-                const bool pred = (sa <= s && s <= sb);
+            sa_[u] = fa;
+            sb_[u] = fbv;
+            sc_[u] = fc;
+            anyf |= ia[u] <= ib[u];
+        }
+        if (__ballot(anyf) == 0ull) continue;  // no band face in this wave's step
+        double gn[UNR];
+        {
+            double e11[UNR], e12[UNR], e22[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int fs = fb.f[u] < m ? fb.f[u] : m - 1;
+                e11[u] = mesh.g11[fs];
+                e12[u] = mesh.g12[fs];
+                e22[u] = mesh.g22[fs];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) gn[u] = grad_norm(fb.d0[u], fb.d1[u], fb.d2[u], e11[u], e12[u], e22[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const bool mem = ia[u] <= ib[u];
+            if (__ballot(mem) == 0ull) continue;
+            const int imin = wave_min_i(mem ? ia[u] : NB);
+            const int imax = wave_max_i(mem ? ib[u] : -1);
+            for (int i = imin; i <= imax; ++i) {
+                const bool pred = mem && ia[u] <= i && i <= ib[u] && b_off[i] >= 0;
                 const uint64_t mask = __ballot(pred);
                 if (mask == 0ull) continue;
                 const int leader = __ffsll((long long)mask) - 1;
                 int slot0 = 0;
-                if (lane == leader) slot0 = atomicAdd(&cnt[s], __popcll(mask));
+                if (lane == leader) slot0 = atomicAdd(&b_cnt[i], __popcll(mask));
                 slot0 = __shfl(slot0, leader, kWave);
                 if (pred) {
                     const int slot = slot0 + __popcll(mask & lanes_below(lane));
-                    if (slot < cap) {
-                        double* bs = band + (int64_t)s * 5 * cap;
-                        bs[slot] = fa;
-                        bs[cap + slot] = fb;
-                        bs[2 * cap + slot] = fc;
-                        bs[3 * cap + slot] = A;
-                        bs[4 * cap + slot] = gn;
+                    if (slot < b_cap[i]) {
+                        double* bs = pool + b_off[i] + slot;
+                        bs[0] = sa_[u];
+                        bs[pstride] = sb_[u];
+                        bs[2 * pstride] = sc_[u];
+                        bs[3 * pstride] = fb.A[u];
+                        bs[4 * pstride] = gn[u];
                     }
                 }
             }
         }
     }
     __syncthreads();
+    if (tid < NB && b_off[tid] >= 0 && b_cnt[tid] > b_cap[tid]) atomicOr(&s_ovf, 2);  // cannot happen: caps are bounds
+    __syncthreads();
 
-    // ---- phase 3: safeguarded Newton per scale (one wave per scale) ------------------------
+    auto band_ref = [&](int i) {
+        BandRef B;
+        B.kL = b_kL[i];
+        B.H = b_H[i];
+        B.full = b_off[i] < 0 || b_cnt[i] > b_cap[i];
+        B.off = B.full ? 0 : b_off[i];
+        B.count = B.full ? 0 : b_cnt[i];
+        return B;
+    };
+
+    // ---- phase 4: safeguarded Newton per scale (one wave per scale) ------------------------
     for (int s = wid; s < S; s += NW) {
         if (act[s] != 1) continue;
         const double T = md.targets[s];
-        const int count = cnt[s];
-        const bool full_scan = count > cap;
-        const double* bs = band + (int64_t)s * 5 * cap;
-        const int klo = lo_k[s], khi = hi_k[s];
-        const double Ain = Hmax[klo];
+        const BandRef B = band_ref(s * (q + 1));
+        const double Ain = Hmax[min(B.kL, nbins)];
+        const int klo = lo_k[s];
         const double br_lo = (double)klo * wb, br_hi = hi_r[s];
+        const int khi = (int)(br_hi / wb + 0.5);
         // initial guess: interpolate the mid-point of the histogram bounds inside the bracket
         double r = 0.5 * (br_lo + br_hi);
         for (int k = klo; k < khi; ++k) {
@@ -506,29 +661,9 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         }
         double r_lo = br_lo, r_hi = br_hi;
         auto eval = [&](double rr, double& A_out, double& dA_out, double& P_out) {
-            double sA = 0.0, sdA = 0.0, sP = 0.0;
-            if (!full_scan) {
-                for (int i = lane; i < count; i += kWave) {
-                    double a, da;
-                    cdf_sorted(bs[i], bs[cap + i], bs[2 * cap + i], bs[3 * cap + i], rr, a, da);
-                    sA += a;
-                    sdA += da;
-                    if (da > 0.0) sP += da * bs[4 * cap + i];
-                }
-            } else {
-                for (int f = lane; f < m; f += kWave) {
-                    const double d0 = d[faces[3 * f]], d1 = d[faces[3 * f + 1]], d2 = d[faces[3 * f + 2]];
-                    if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) continue;
-                    double fa = d0, fb = d1, fc = d2;
-                    sort3(fa, fb, fc);
-                    if (!(bin_of(fc, wb, nbins) >= klo && fa < br_hi)) continue;
-                    double a, da;
-                    cdf_sorted(fa, fb, fc, mesh.larea[f], rr, a, da);
-                    sA += a;
-                    sdA += da;
-                    if (da > 0.0) sP += da * grad_norm(d0, d1, d2, mesh.g11[f], mesh.g12[f], mesh.g22[f]);
-                }
-            }
+            double sA, sdA, sP;
+            band_sums(B, rr, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
+                      sdA, sP);
             A_out = Ain + wave_sum(sA);
             dA_out = wave_sum(sdA);
             P_out = wave_sum(sP);
@@ -573,94 +708,55 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     }
     __syncthreads();
 
-    // ---- phase 4: supplementary samples between consecutive radii --------------------------
-    if (tid < NE) {
-        const int s = tid / q, jj = tid - s * q;
+    // ---- phase 5: supplementary samples between consecutive radii --------------------------
+    for (int e = tid; e < NE; e += NT) {
+        const int s = e / q, jj = e - s * q;
         const double r0 = r_out[s], r1 = r_out[s + 1];
         double rr = NAN;
-        if (r0 == r0 && r1 == r1 && r0 > 0.0 && r1 > r0) rr = exp(log(r0) + md.fracs[jj] * (log(r1) - log(r0)));
-        rho[tid] = rr;
+        if (r0 == r0 && r1 == r1 && r0 > 0.0 && r1 > r0) rr = log_mix(r0, r1, md.fracs[jj]);
+        U.x.rho[e] = rr;
+        U.x.acc[e] = NAN;
     }
     __syncthreads();
-    if (tid == 0) {
-        int nv = 0;
-        for (int e = 0; e < NE; ++e) {
-            const double rr = rho[e];
-            if (rr == rr) {
-                ridx[nv] = e;
-                vr[nv] = rr;
-                rk[nv] = (int)(rr / wb);
-                ++nv;
-            }
+    for (int e = wid; e < NE; e += NW) {
+        const double rr = U.x.rho[e];
+        if (rr != rr) continue;
+        const int s = e / q, jj = e - s * q;
+        BandRef B = band_ref(s * (q + 1) + 1 + jj);
+        if (!((double)B.kL * wb <= rr && rr <= B.H)) {
+            // outside its band (cannot happen with the slack above): every face, own bracket
+            B.kL = min(bin_of(rr, wb, nbins), (int)(rr / wb));
+            B.H = rr;
+            B.full = true;
+            if (lane == 0) atomicOr(&s_ovf, 4);
         }
-        s_nv = nv;
-    }
-    __syncthreads();
-    const int nv = s_nv;
-    for (int v = tid; v < nv; v += NT) racc[v] = (rk[v] >= nbins) ? total : Hmax[rk[v]];
-    __syncthreads();
-    if (nv > 0) {
-        for (int base0 = wid * kWave * UNR; base0 < m; base0 += NT * UNR) {
-            FaceBatch fbat;
-            load_faces(faces, d, m, base0, lane, fbat);
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int f = fbat.f[u];
-                int va = nv, vb = -1;
-                double fa = fbat.d0[u], fb = fbat.d1[u], fc = fbat.d2[u], A = 0;
-                if (f < m && isfinite(fa) && isfinite(fb) && isfinite(fc)) {
-                    sort3(fa, fb, fc);
-                    const int bh = bin_of(fc, wb, nbins);
-                    int lo = 0, hi = nv;  // first v with vr > fa
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (vr[mid] > fa) hi = mid; else lo = mid + 1;
-                    }
-                    va = lo;
-                    lo = 0;
-                    hi = nv;  // first v with rk > bh
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (rk[mid] > bh) hi = mid; else lo = mid + 1;
-                    }
-                    vb = lo - 1;
-                    A = mesh.larea[f];
-                }
-                if (__ballot(va <= vb) == 0ull) continue;  // no face straddles a sample radius
-                const int vmin = wave_min_i(va <= vb ? va : nv);
-                const int vmax = wave_max_i(va <= vb ? vb : -1);
-                for (int v = vmin; v <= vmax; ++v) {
-                    double a = 0.0, da;
-                    if (va <= v && v <= vb && rk[v] < nbins) cdf_sorted(fa, fb, fc, A, vr[v], a, da);
-                    const double sum = wave_sum(a);
-                    if (lane == 0 && sum != 0.0) atomicAdd(&racc[v], sum);
-                }
-            }
-        }
+        double sA, sdA, sP;
+        band_sums(B, rr, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA, sdA,
+                  sP);
+        const double A = Hmax[min(B.kL, nbins)] + wave_sum(sA);
+        if (lane == 0) U.x.acc[e] = A;
     }
     __syncthreads();
 
-    // ---- phase 5: outputs and cleaned (radius, area) samples -------------------------------
+    // ---- phase 6: outputs and cleaned (radius, area) samples -------------------------------
     for (int s = tid; s < S; s += NT) {
         radii[s] = r_out[s];
         perim[s] = (r_out[s] == r_out[s]) ? (act[s] == 1 ? p_out[s] : 0.0) : NAN;
     }
     for (int e = tid; e < NE; e += NT) {
-        er[e] = rho[e];
-        ea[e] = NAN;
+        er[e] = U.x.rho[e];
+        ea[e] = U.x.acc[e];
     }
-    __syncthreads();
-    for (int v = tid; v < nv; v += NT) ea[ridx[v]] = racc[v];
-    __syncthreads();
     if (tid == 0) {
-        int ne = 0;
+        int ne = 0, nv = 0;
         for (int s = 0; s < S; ++s) ne += ev_out[s];
+        for (int e = 0; e < NE; ++e) nv += (U.x.rho[e] == U.x.rho[e]) ? 1 : 0;
         ne += nv;
         const int maxp = S + NE;
         float* sr = out.samp_r + (int64_t)j * maxp;
         float* sa = out.samp_a + (int64_t)j * maxp;
-        double* pr = pair_r;
-        double* pa = pair_a;
+        double* pr = U.x.pair_r;
+        double* pa = U.x.pair_a;
         int np = 0;
         for (int s = 0; s < S; ++s)
             if (isfinite(r_out[s])) {
@@ -669,8 +765,8 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                 ++np;
             }
         const double db32 = (double)(float)dist_b;  // the reference reads the float32-stored value back
-        for (int v = 0; v < nv; ++v) {
-            const double rr = vr[v], aa = racc[v];
+        for (int e = 0; e < NE; ++e) {
+            const double rr = U.x.rho[e], aa = U.x.acc[e];
             if (!(isfinite(rr) && isfinite(aa))) continue;
             if (md.cap >= 0.0) {
                 double loss;
@@ -715,8 +811,6 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             msd_u = v4[0] / (double)i4[2];
             msd_w = (isfinite(v4[2]) && v4[2] > 0.0) ? v4[1] / v4[2] : NAN;
         }
-        int ovf = 0;
-        for (int s = 0; s < S; ++s) ovf |= (act[s] == 1 && cnt[s] > cap);
         scal[kSlotFlag] = bad ? 1.0 : 0.0;
         scal[kSlotNeg] = neg_frac;
         scal[kSlotViol] = viol_frac;
@@ -725,7 +819,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         scal[kSlotMsdW] = msd_w;
         scal[kSlotNEval] = (double)ne;
         scal[kSlotNSamples] = (double)nout;
-        scal[kSlotOverflow] = (double)ovf;
+        scal[kSlotOverflow] = (double)s_ovf;
     }
 }
 

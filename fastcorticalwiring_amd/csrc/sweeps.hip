@@ -38,7 +38,8 @@ static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling a
 
 enum class Sweep { Forward, Backward };
 
-template <Sweep MODE, bool IDX_LDS>
+// KIDS = false: every node of the launch is a leaf (no child update rows to gather).
+template <Sweep MODE, bool IDX_LDS, bool KIDS>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         if (MODE == Sweep::Backward) {
             // source row of every depth index: own rows as positions, boundary rows as -(pos + 1)
             for (int i = tid; i < f; i += kThreads) sidx[i] = i < s ? start + i : -(B[i - s] + 1);
-        } else {
+        } else if (KIDS) {
             for (int i = tid; i < 2 * s; i += kThreads) sidx[i] = cown[2 * start + i];
         }
     }
@@ -100,15 +101,16 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 // F rows: own RHS row minus up to two children's update rows
                 const int lc = min(l, s - 1);
                 const int pos = start + lc;
-                const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * pos];
-                const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * pos + 1];
-                const double* xr = Bsrc + (int64_t)pos * ldk;
                 keep_b |= (l < s) ? (1u << i) : 0u;
-                bit_u |= (ca >= 0) ? (1u << i) : 0u;
-                bit_v |= (cb >= 0) ? (1u << i) : 0u;
-                rb[i] = xr[c];
-                ru[i] = U[(int64_t)max(ca, 0) * ldk + c];  // update row 0 stands in when unused
-                rv[i] = U[(int64_t)max(cb, 0) * ldk + c];
+                rb[i] = Bsrc[(int64_t)pos * ldk + c];
+                if (KIDS) {
+                    const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * pos];
+                    const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * pos + 1];
+                    bit_u |= (ca >= 0) ? (1u << i) : 0u;
+                    bit_v |= (cb >= 0) ? (1u << i) : 0u;
+                    ru[i] = U[(int64_t)max(ca, 0) * ldk + c];  // update row 0 stands in when unused
+                    rv[i] = U[(int64_t)max(cb, 0) * ldk + c];
+                }
             } else {
                 // [Y_S ; -Z_B] rows: own rows from Y, boundary rows negated from Z
                 const int lc = min(l, f - 1);
@@ -142,8 +144,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         for (int i = 0; i < kLoadsB; ++i) {
             const int e = tid + i * kThreads;
             double v;
-            if (MODE == Sweep::Forward)
+            if (MODE == Sweep::Forward && KIDS)
                 v = rb[i] - (((bit_u >> i) & 1u) ? ru[i] : 0.0) - (((bit_v >> i) & 1u) ? rv[i] : 0.0);
+            else if (MODE == Sweep::Forward)
+                v = rb[i];
             else
                 v = ((bit_u >> i) & 1u) ? -rb[i] : rb[i];
             Bs[e / kTileN][e % kTileN] = ((keep_b >> i) & 1u) ? v : 0.0;
@@ -211,17 +215,21 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds) {
+                     double* U, bool idx_lds, bool leaves) {
     if (ntiles <= 0) return;
     const dim3 grid(ntiles, ldk / kTileN);
-    if (idx_lds)
-        sweep_kernel<Sweep::Forward, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, nullptr,
-                                                                      nullptr, node_poff, node_uoff, contrib_own,
-                                                                      contrib_tail, panel, X, nullptr, Y, U);
+#define FCW_FWD(LDS, KIDS)                                                                                      \
+    sweep_kernel<Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, \
+                                                                       nullptr, nullptr, node_poff, node_uoff,  \
+                                                                       contrib_own, contrib_tail, panel, X,     \
+                                                                       nullptr, Y, U)
+    if (leaves)
+        FCW_FWD(true, false);
+    else if (idx_lds)
+        FCW_FWD(true, true);
     else
-        sweep_kernel<Sweep::Forward, false><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, nullptr,
-                                                                       nullptr, node_poff, node_uoff, contrib_own,
-                                                                       contrib_tail, panel, X, nullptr, Y, U);
+        FCW_FWD(false, true);
+#undef FCW_FWD
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -230,13 +238,14 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
     if (ntiles <= 0) return;
     const dim3 grid(ntiles, ldk / kTileN);
     if (idx_lds)
-        sweep_kernel<Sweep::Backward, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
-                                                                       node_boff, bidx, node_poff, nullptr, nullptr,
-                                                                       nullptr, panel, Y, Z, Z, nullptr);
+        sweep_kernel<Sweep::Backward, true, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
+                                                                             node_boff, bidx, node_poff, nullptr,
+                                                                             nullptr, nullptr, panel, Y, Z, Z, nullptr);
     else
-        sweep_kernel<Sweep::Backward, false><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
-                                                                        node_boff, bidx, node_poff, nullptr, nullptr,
-                                                                        nullptr, panel, Y, Z, Z, nullptr);
+        sweep_kernel<Sweep::Backward, false, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
+                                                                              node_boff, bidx, node_poff, nullptr,
+                                                                              nullptr, nullptr, panel, Y, Z, Z,
+                                                                              nullptr);
 }
 
 }  // namespace fcw
