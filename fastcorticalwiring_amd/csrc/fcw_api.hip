@@ -474,7 +474,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->host_threads = 0;
     opt->device = 0;
     opt->ir_steps = 2;
-    opt->streams = 2;
+    opt->streams = 4;
     opt->sparse_rhs = 1;
 }
 

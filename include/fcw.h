@@ -50,7 +50,7 @@ typedef struct {
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
     int ir_steps;          /* refinement sweeps per system in FCW_FP32_IR mode; default 2 */
-    int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 2 */
+    int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 4 */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
     int reserved[4];
 } fcw_options;
