@@ -12,7 +12,7 @@
 //            radii it lies between); each band gets an exact-size slot range in
 //            the per-source scratch.
 //   phase 3  face pass: every face whose [dmin, dmax] meets a band is copied,
-//            pre-sorted, into that band. For r in [kL wb, H],
+//            pre-sorted (with its area and index), into that band. For r in [kL wb, H],
 //            A(r) = (area of faces with dmax bin < kL) + sum over the band of the
 //            face's exact area inside r.
 //   phase 4  safeguarded Newton on the exact piecewise-quadratic A(r) (one wave
@@ -179,9 +179,10 @@ __device__ void block_reduce(Red& sm, double (&v)[4], int (&iv)[4], int smax, in
 __device__ inline double log_mix(double r0, double r1, double fr) { return exp(log(r0) + fr * (log(r1) - log(r0))); }
 
 // One band's sums at radius r over lanes of one wave: area inside r beyond the fully-inside
-// part, its derivative, and the coarea perimeter density. `pool` holds the band's sorted
-// (a, b, c), area and |grad d| in five arrays of stride `pstride`. With `full`, the band is
-// instead recomputed from every face (the band did not fit in the scratch).
+// part and its derivative, and (with want_p) the coarea perimeter density. `pool` holds the
+// band's sorted (a, b, c), area and face index in five arrays of stride `pstride`; |grad d| is
+// recomputed from the face for the faces that cross r. With `full`, the band is instead
+// recomputed from every face (the band did not fit in the scratch).
 struct BandRef {
     int kL;
     double H;
@@ -189,7 +190,7 @@ struct BandRef {
     bool full;
 };
 
-__device__ void band_sums(const BandRef& B, double r, const double* __restrict__ pool, int64_t pstride,
+__device__ void band_sums(const BandRef& B, double r, bool want_p, const double* __restrict__ pool, int64_t pstride,
                           const int* __restrict__ faces, const double* __restrict__ d,
                           const double* __restrict__ larea, const double* __restrict__ g11,
                           const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
@@ -202,9 +203,9 @@ __device__ void band_sums(const BandRef& B, double r, const double* __restrict__
         const double* pb = pa + pstride;
         const double* pc = pb + pstride;
         const double* pA = pc + pstride;
-        const double* pg = pA + pstride;
+        const double* pf = pA + pstride;
         for (int i0 = lane; i0 < B.count; i0 += U * kWave) {
-            double a[U], b[U], c[U], A[U], g[U];
+            double a[U], b[U], c[U], A[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = min(i0 + u * kWave, B.count - 1);
@@ -212,17 +213,42 @@ __device__ void band_sums(const BandRef& B, double r, const double* __restrict__
                 b[u] = pb[i];
                 c[u] = pc[i];
                 A[u] = pA[i];
-                g[u] = pg[i];
             }
             __builtin_amdgcn_sched_barrier(0);
+            double dx[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                double x, dx;
-                cdf_sorted(a[u], b[u], c[u], A[u], r, x, dx);
+                double x;
+                cdf_sorted(a[u], b[u], c[u], A[u], r, x, dx[u]);
                 const bool ok = i0 + u * kWave < B.count;
                 sA += ok ? x : 0.0;
-                sdA += ok ? dx : 0.0;
-                sP += (ok && dx > 0.0) ? dx * g[u] : 0.0;
+                sdA += ok ? dx[u] : 0.0;
+                dx[u] = ok ? dx[u] : 0.0;
+            }
+            if (want_p) {
+                int f[U], i1[U], i2[U], i3[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) f[u] = (int)pf[min(i0 + u * kWave, B.count - 1)];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    i1[u] = faces[3 * f[u]];
+                    i2[u] = faces[3 * f[u] + 1];
+                    i3[u] = faces[3 * f[u] + 2];
+                }
+                double d0[U], d1[U], d2[U], e11[U], e12[U], e22[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    d0[u] = d[i1[u]];
+                    d1[u] = d[i2[u]];
+                    d2[u] = d[i3[u]];
+                    e11[u] = g11[f[u]];
+                    e12[u] = g12[f[u]];
+                    e22[u] = g22[f[u]];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    sP += (dx[u] > 0.0) ? dx[u] * grad_norm(d0[u], d1[u], d2[u], e11[u], e12[u], e22[u]) : 0.0;
             }
         }
     } else {
@@ -236,7 +262,7 @@ __device__ void band_sums(const BandRef& B, double r, const double* __restrict__
             cdf_sorted(fa, fb, fc, larea[f], r, x, dx);
             sA += x;
             sdA += dx;
-            if (dx > 0.0) sP += dx * grad_norm(d0, d1, d2, g11[f], g12[f], g22[f]);
+            if (want_p && dx > 0.0) sP += dx * grad_norm(d0, d1, d2, g11[f], g12[f], g22[f]);
         }
     }
 }
@@ -269,8 +295,13 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             unsigned cmin[kMaxBins + 1];
         } c;
         struct {
+            int lk[kMaxBins + 1];  // per dmax bin: last band whose kL <= bin
+            int fa[kMaxBins + 1];  // per dmin bin: first band whose H > bin start
+        } t;
+        struct {
             double rho[kMaxExtras], acc[kMaxExtras];
             double pair_r[kMaxBands], pair_a[kMaxBands];
+            int key[kMaxBands], pos[kMaxBands];
         } x;
     } U;
     __shared__ int b_kL[kMaxBands], b_cnt[kMaxBands], b_off[kMaxBands], b_cap[kMaxBands];
@@ -279,6 +310,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __shared__ double hi_r[kMaxScales], r_out[kMaxScales], p_out[kMaxScales];
     __shared__ int s_nbins, s_ovf;
     __shared__ double s_wb;
+    __shared__ double s_scal[kSlotCount];
 
     // ---- phase 0: health screen, MSD, distance to boundary -------------------------------
     const int sp = src_pos[j];
@@ -336,6 +368,19 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const double viol_frac = i4[3] ? NAN : (double)i4[1] / n;
     const bool bad = (i4[3] > 0) || neg_frac > 1e-3 || viol_frac > 1e-3;
     const double dist_b = (md.nb > 0) ? (bi4[0] > 0 ? b4[0] : NAN) : INFINITY;
+    if (tid == 0) {  // parked in LDS until the end (frees registers for the face passes)
+        s_scal[kSlotFlag] = bad ? 1.0 : 0.0;
+        s_scal[kSlotNeg] = neg_frac;
+        s_scal[kSlotViol] = viol_frac;
+        s_scal[kSlotDistB] = dist_b;
+        double msd_u = NAN, msd_w = NAN;
+        if (i4[2] > 0) {
+            msd_u = v4[0] / (double)i4[2];
+            msd_w = (isfinite(v4[2]) && v4[2] > 0.0) ? v4[1] / v4[2] : NAN;
+        }
+        s_scal[kSlotMsdU] = msd_u;
+        s_scal[kSlotMsdW] = msd_w;
+    }
     if (health_mode && bad) {
         if (tid == 0) {
             scal[kSlotFlag] = 1.0;
@@ -548,6 +593,25 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         s_ovf = ovf;
     }
     __syncthreads();
+    // Band lookup by bin (the count histograms are no longer needed): a face's bands are
+    // [first band with H > dmin, last band with kL <= dmax bin].
+    for (int k = tid; k < nbins; k += NT) {
+        int lo = 0, hi = NB;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (b_kL[mid] > k) hi = mid; else lo = mid + 1;
+        }
+        U.t.lk[k] = lo - 1;
+        const double x = (double)k * wb;
+        lo = 0;
+        hi = NB;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (b_H[mid] > x) hi = mid; else lo = mid + 1;
+        }
+        U.t.fa[k] = lo;
+    }
+    __syncthreads();
 
     // ---- phase 3: band faces, pre-sorted, contiguous per band -----------------------------
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
@@ -563,20 +627,10 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             double fa = fb.d0[u], fbv = fb.d1[u], fc = fb.d2[u];
             if (fb.f[u] < m && isfinite(fa) && isfinite(fbv) && isfinite(fc)) {
                 sort3(fa, fbv, fc);
-                const int bh = bin_of(fc, wb, nbins);
-                int lo = 0, hi = NB;  // first band with H > dmin
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (b_H[mid] > fa) hi = mid; else lo = mid + 1;
-                }
-                ia[u] = lo;
-                lo = 0;
-                hi = NB;  // first band with kL > dmax bin
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (b_kL[mid] > bh) hi = mid; else lo = mid + 1;
-                }
-                ib[u] = lo - 1;
+                ib[u] = U.t.lk[bin_of(fc, wb, nbins)];
+                int i = U.t.fa[bin_of(fa, wb, nbins)];  // H > bin start; step past H <= dmin
+                while (i < NB && b_H[i] <= fa) ++i;
+                ia[u] = i;
             }
             sa_[u] = fa;
             sb_[u] = fbv;
@@ -584,20 +638,6 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             anyf |= ia[u] <= ib[u];
         }
         if (__ballot(anyf) == 0ull) continue;  // no band face in this wave's step
-        double gn[UNR];
-        {
-            double e11[UNR], e12[UNR], e22[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int fs = fb.f[u] < m ? fb.f[u] : m - 1;
-                e11[u] = mesh.g11[fs];
-                e12[u] = mesh.g12[fs];
-                e22[u] = mesh.g22[fs];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) gn[u] = grad_norm(fb.d0[u], fb.d1[u], fb.d2[u], e11[u], e12[u], e22[u]);
-        }
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const bool mem = ia[u] <= ib[u];
@@ -620,7 +660,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                         bs[pstride] = sb_[u];
                         bs[2 * pstride] = sc_[u];
                         bs[3 * pstride] = fb.A[u];
-                        bs[4 * pstride] = gn[u];
+                        bs[4 * pstride] = (double)fb.f[u];
                     }
                 }
             }
@@ -660,26 +700,25 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             }
         }
         double r_lo = br_lo, r_hi = br_hi;
-        auto eval = [&](double rr, double& A_out, double& dA_out, double& P_out) {
+        auto eval = [&](double rr, bool want_p, double& A_out, double& dA_out, double& P_out) {
             double sA, sdA, sP;
-            band_sums(B, rr, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
-                      sdA, sP);
+            band_sums(B, rr, want_p, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb,
+                      nbins, sA, sdA, sP);
             A_out = Ain + wave_sum(sA);
             dA_out = wave_sum(sdA);
             P_out = wave_sum(sP);
         };
         int n_eval = 0;
         double result = r, Pres = 0.0;
-        bool have_p = false;
+        bool converged = false;
         for (int it = 0; it < md.max_iter; ++it) {
             double A, dA, Pv;
-            eval(r, A, dA, Pv);
+            eval(r, false, A, dA, Pv);
             ++n_eval;
             const double err = A - T;
             if (fabs(err) <= md.rtol * T) {
                 result = r;
-                Pres = Pv;
-                have_p = true;
+                converged = true;
                 break;
             }
             if (err > 0.0) r_hi = r; else r_lo = r;
@@ -695,10 +734,11 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             r = rn;
             result = r;
         }
-        if (!have_p) {
+        {
+            // perimeter at the root (counted as an evaluation when the root was not the last one)
             double A, dA;
-            eval(result, A, dA, Pres);
-            ++n_eval;
+            eval(result, true, A, dA, Pres);
+            if (!converged) ++n_eval;
         }
         if (lane == 0) {
             r_out[s] = result;
@@ -731,8 +771,8 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             if (lane == 0) atomicOr(&s_ovf, 4);
         }
         double sA, sdA, sP;
-        band_sums(B, rr, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA, sdA,
-                  sP);
+        band_sums(B, rr, false, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
+                  sdA, sP);
         const double A = Hmax[min(B.kL, nbins)] + wave_sum(sA);
         if (lane == 0) U.x.acc[e] = A;
     }
@@ -747,76 +787,89 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         er[e] = U.x.rho[e];
         ea[e] = U.x.acc[e];
     }
+    // Cleaned samples (core_analysis.py:1599-1609, 1424-1439) by wave 0: candidates are the
+    // solved (radius, target) pairs then the kept supplementary pairs, stably sorted by radius
+    // (rank = smaller radii + equal radii earlier), then radii within tol of their predecessor
+    // dropped.
+    const int maxp = S + NE;
+    if (wid == 0) {
+        const double db32 = (double)(float)s_scal[kSlotDistB];  // the reference reads the float32-stored value back
+        int np = 0;
+        for (int c0 = 0; c0 < maxp; c0 += kWave) {
+            const int c = c0 + lane;
+            bool keep = false;
+            double rr = 0.0, aa = 0.0;
+            if (c < S) {
+                rr = r_out[c];
+                aa = md.targets[c];
+                keep = isfinite(rr);
+            } else if (c < maxp) {
+                rr = U.x.rho[c - S];
+                aa = U.x.acc[c - S];
+                keep = isfinite(rr) && isfinite(aa);
+                if (keep && md.cap >= 0.0) {
+                    double loss;
+                    if (rr <= 0.0) {
+                        loss = NAN;
+                    } else if (!isfinite(db32) || db32 >= rr) {
+                        loss = 0.0;
+                    } else {
+                        const double x = fmin(1.0, fmax(0.0, db32 / rr));
+                        loss = (acos(x) - x * sqrt(fmax(0.0, 1.0 - x * x))) / M_PI;
+                    }
+                    keep = !(loss > md.cap);
+                }
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) {
+                const int at = np + __popcll(km & lanes_below(lane));
+                U.x.pair_r[at] = rr;
+                U.x.pair_a[at] = aa;
+            }
+            np += __popcll(km);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int a = lane; a < np; a += kWave) {
+            const double ra = U.x.pair_r[a];
+            int rank = 0;
+            for (int b = 0; b < np; ++b) {
+                const double rb = U.x.pair_r[b];
+                rank += (rb < ra || (rb == ra && b < a)) ? 1 : 0;
+            }
+            U.x.pos[rank] = a;
+        }
+        __builtin_amdgcn_wave_barrier();
+        float* sr = out.samp_r + (int64_t)j * maxp;
+        float* sa = out.samp_a + (int64_t)j * maxp;
+        const double tol = np > 0 ? fmax(1e-12, 1e-9 * (U.x.pair_r[U.x.pos[np - 1]] - U.x.pair_r[U.x.pos[0]])) : 0.0;
+        int nout = 0;
+        for (int c0 = 0; c0 < np; c0 += kWave) {
+            const int c = c0 + lane;
+            bool keep = false;
+            double rr = 0.0, aa = 0.0;
+            if (c < np) {
+                rr = U.x.pair_r[U.x.pos[c]];
+                aa = U.x.pair_a[U.x.pos[c]];
+                keep = (c == 0) || (rr - U.x.pair_r[U.x.pos[c - 1]]) > tol;
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) {
+                const int at = nout + __popcll(km & lanes_below(lane));
+                sr[at] = (float)rr;
+                sa[at] = (float)aa;
+            }
+            nout += __popcll(km);
+        }
+        if (lane == 0) U.x.key[0] = nout;
+    }
+    __syncthreads();
     if (tid == 0) {
         int ne = 0, nv = 0;
         for (int s = 0; s < S; ++s) ne += ev_out[s];
         for (int e = 0; e < NE; ++e) nv += (U.x.rho[e] == U.x.rho[e]) ? 1 : 0;
         ne += nv;
-        const int maxp = S + NE;
-        float* sr = out.samp_r + (int64_t)j * maxp;
-        float* sa = out.samp_a + (int64_t)j * maxp;
-        double* pr = U.x.pair_r;
-        double* pa = U.x.pair_a;
-        int np = 0;
-        for (int s = 0; s < S; ++s)
-            if (isfinite(r_out[s])) {
-                pr[np] = r_out[s];
-                pa[np] = md.targets[s];
-                ++np;
-            }
-        const double db32 = (double)(float)dist_b;  // the reference reads the float32-stored value back
-        for (int e = 0; e < NE; ++e) {
-            const double rr = U.x.rho[e], aa = U.x.acc[e];
-            if (!(isfinite(rr) && isfinite(aa))) continue;
-            if (md.cap >= 0.0) {
-                double loss;
-                if (rr <= 0.0) {
-                    loss = NAN;
-                } else if (!isfinite(db32) || db32 >= rr) {
-                    loss = 0.0;
-                } else {
-                    const double x = fmin(1.0, fmax(0.0, db32 / rr));
-                    loss = (acos(x) - x * sqrt(fmax(0.0, 1.0 - x * x))) / M_PI;
-                }
-                if (loss > md.cap) continue;
-            }
-            pr[np] = rr;
-            pa[np] = aa;
-            ++np;
-        }
-        for (int a = 1; a < np; ++a) {  // stable insertion sort by radius
-            const double kr = pr[a], ka = pa[a];
-            int b = a - 1;
-            while (b >= 0 && pr[b] > kr) {
-                pr[b + 1] = pr[b];
-                pa[b + 1] = pa[b];
-                --b;
-            }
-            pr[b + 1] = kr;
-            pa[b + 1] = ka;
-        }
-        int nout = 0;
-        if (np > 0) {
-            const double tol = fmax(1e-12, 1e-9 * (pr[np - 1] - pr[0]));
-            for (int a = 0; a < np; ++a) {
-                if (a == 0 || (pr[a] - pr[a - 1]) > tol) {
-                    sr[nout] = (float)pr[a];
-                    sa[nout] = (float)pa[a];
-                    ++nout;
-                }
-            }
-        }
-        double msd_u = NAN, msd_w = NAN;
-        if (i4[2] > 0) {
-            msd_u = v4[0] / (double)i4[2];
-            msd_w = (isfinite(v4[2]) && v4[2] > 0.0) ? v4[1] / v4[2] : NAN;
-        }
-        scal[kSlotFlag] = bad ? 1.0 : 0.0;
-        scal[kSlotNeg] = neg_frac;
-        scal[kSlotViol] = viol_frac;
-        scal[kSlotDistB] = dist_b;
-        scal[kSlotMsdU] = msd_u;
-        scal[kSlotMsdW] = msd_w;
+        const int nout = U.x.key[0];
+        for (int c = kSlotFlag; c <= kSlotMsdW; ++c) scal[c] = s_scal[c];
         scal[kSlotNEval] = (double)ne;
         scal[kSlotNSamples] = (double)nout;
         scal[kSlotOverflow] = (double)s_ovf;
