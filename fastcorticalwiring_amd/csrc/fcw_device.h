@@ -25,7 +25,8 @@ constexpr int kSweepIdxCap = 1024;
 // Metric kernel limits.
 constexpr int kMaxScales = 32;
 constexpr int kMaxExtras = 256;
-constexpr int kMaxBins = 2048;
+constexpr int kMaxBins = 1536;
+constexpr int kMaxChunks = 2048;  // 256-face chunks with recorded distance bounds (larger meshes: no skipping)
 constexpr int kMetricThreads = 1024;
 
 // Per-source scalar outputs, in this order (double each).

@@ -130,6 +130,18 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // The 1-D grid is laid out so that the column slices of one row group are dispatched to the
 // same XCD (workgroups are handed to the 8 XCDs round robin), where they share the staged
 // face data in that XCD's L2.
+// 1/sqrt(q) for q > 0: the hardware estimate refined by two Newton steps (relative error a few
+// units in the last place), with q scaled out of the range where the estimate loses accuracy.
+// It replaces a correctly rounded square root followed by a division.
+__device__ inline double rsqrt_refined(double q) {
+    const bool tiny = q < 0x1p-767;
+    const double qs = tiny ? q * 0x1p+256 : q;
+    double y = __builtin_amdgcn_rsq(qs);
+    y = fma(0.5 * y, fma(-qs * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-qs * y, y, 1.0), y);
+    return tiny ? y * 0x1p+128 : y;
+}
+
 constexpr int kDivRows = 16;   // rows per workgroup (4 per wave)
 constexpr int kDivInc = 256;   // incidences staged per pass (a pass per 256 when rows are long)
 constexpr int kDivC = 6;       // incidences in flight per row step (interior vertices have about 6)
@@ -202,10 +214,10 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
                     const double gx = du1 * G[3] + du2 * G[6];
                     const double gy = du1 * G[4] + du2 * G[7];
                     const double gz = du1 * G[5] + du2 * G[8];
-                    const double nn = sqrt(gx * gx + gy * gy + gz * gz);
-                    const double sc = -G[9] / nn;
-                    const double term = sc * (gx * G[0] + gy * G[1] + gz * G[2]);
-                    acc[i] += (e + q < re && nn > 1e-300) ? term : 0.0;
+                    const double qq = gx * gx + gy * gy + gz * gz;
+                    const double term = -G[9] * (gx * G[0] + gy * G[1] + gz * G[2]) * rsqrt_refined(qq);
+                    // |grad u| > 1e-300 (fastcw_lean.py:433) holds for every positive qq
+                    acc[i] += (e + q < re && qq > 0.0) ? term : 0.0;
                 }
             }
         }

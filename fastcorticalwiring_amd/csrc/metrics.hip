@@ -12,7 +12,10 @@
 //            radii it lies between); each band gets an exact-size slot range in
 //            the per-source scratch.
 //   phase 3  face pass: every face whose [dmin, dmax] meets a band is copied,
-//            pre-sorted (with its area and index), into that band. For r in [kL wb, H],
+//            pre-sorted (with its area and index), into that band. Phase 1
+//            records the distance range of every 256-face chunk, so chunks that
+//            meet no band (most of the surface: the bands lie within the largest
+//            radius) are skipped without loading them. For r in [kL wb, H],
 //            A(r) = (area of faces with dmax bin < kL) + sum over the band of the
 //            face's exact area inside r.
 //   phase 4  safeguarded Newton on the exact piecewise-quadratic A(r) (one wave
@@ -308,7 +311,10 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __shared__ double b_H[kMaxBands];
     __shared__ int lo_k[kMaxScales], act[kMaxScales], ev_out[kMaxScales];
     __shared__ double hi_r[kMaxScales], r_out[kMaxScales], p_out[kMaxScales];
+    __shared__ float ch_lo[kMaxChunks], ch_hi[kMaxChunks];  // per 256-face chunk: min dmin (rounded
+                                                             // down), max dmax (rounded up)
     __shared__ int s_nbins, s_ovf;
+    __shared__ double s_skip_lo, s_skip_hi;
     __shared__ double s_wb;
     __shared__ double s_scal[kSlotCount];
 
@@ -425,19 +431,36 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
     const double wb = s_wb;
     const int nbins = s_nbins;
+    const bool chunked = m <= kMaxChunks * kWave * UNR;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         FaceBatch fb;
         load_faces(faces, mesh.larea, d, m, base, lane, fb);
+        double cmin = INFINITY, cmax = -INFINITY;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const double d0 = fb.d0[u], d1 = fb.d1[u], d2 = fb.d2[u];
             if (fb.f[u] < m && isfinite(d0) && isfinite(d1) && isfinite(d2)) {
-                const int kh = bin_of(fmax(d0, fmax(d1, d2)), wb, nbins);
-                const int kl = bin_of(fmin(d0, fmin(d1, d2)), wb, nbins);
+                const double hi = fmax(d0, fmax(d1, d2)), lo = fmin(d0, fmin(d1, d2));
+                const int kh = bin_of(hi, wb, nbins);
+                const int kl = bin_of(lo, wb, nbins);
                 atomicAdd(&Hmax[kh], fb.A[u]);
                 atomicAdd(&Hmin[kl], fb.A[u]);
                 atomicAdd(&U.c.cmax[kh], 1u);
                 atomicAdd(&U.c.cmin[kl], 1u);
+                cmin = fmin(cmin, lo);
+                cmax = fmax(cmax, hi);
+            }
+        }
+        if (chunked) {
+            // float bounds rounded outwards, reduced over the wave
+            float flo = __double2float_rd(cmin), fhi = __double2float_ru(cmax);
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                flo = fminf(flo, __shfl_xor(flo, o, kWave));
+                fhi = fmaxf(fhi, __shfl_xor(fhi, o, kWave));
+            }
+            if (lane == 0) {
+                ch_lo[base / (kWave * UNR)] = flo;
+                ch_hi[base / (kWave * UNR)] = fhi;
             }
         }
     }
@@ -591,6 +614,16 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             b_cnt[i] = 0;
         }
         s_ovf = ovf;
+        // a chunk can hold a band face only if some face has dmax bin >= min kL and dmin < max H
+        // over the bands that can hold faces (one bin of slack on the kL side)
+        double lo = INFINITY, hi = -INFINITY;
+        for (int i = 0; i < NB; ++i) {
+            if (b_kL[i] >= nbins || !(b_H[i] > -INFINITY)) continue;
+            lo = fmin(lo, (double)(b_kL[i] - 1) * wb);
+            hi = fmax(hi, b_H[i]);
+        }
+        s_skip_lo = lo;
+        s_skip_hi = hi;
     }
     __syncthreads();
     // Band lookup by bin (the count histograms are no longer needed): a face's bands are
@@ -614,7 +647,12 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
 
     // ---- phase 3: band faces, pre-sorted, contiguous per band -----------------------------
+    const double skip_lo = s_skip_lo, skip_hi = s_skip_hi;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+        if (chunked) {
+            const int c = base / (kWave * UNR);
+            if ((double)ch_hi[c] < skip_lo || (double)ch_lo[c] >= skip_hi) continue;  // meets no band
+        }
         FaceBatch fb;
         load_faces(faces, mesh.larea, d, m, base, lane, fb);
         int ia[UNR], ib[UNR];
