@@ -92,6 +92,8 @@ struct BlockPlan {
 struct Workspace {
     hipStream_t stream = nullptr;
     DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
+    DBuf<double> cpart;              // chunk_stats records (k x nch x kChunkStats)
+    DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
     DBuf<int> src_pos;
     std::vector<EventPair> ev_pool;
     std::vector<std::pair<int, EventPair>> ev_pending;
@@ -121,6 +123,10 @@ struct fcw_ctx {
     // per-call plans
     DBuf<fcw::LevelTile> plan_tiles;
     DBuf<int> plan_nodes;
+    // 256-face chunks: vertices (owned first, cnown of them), area
+    int nch = 0;
+    DBuf<int> cvptr, cverts, cnown, isb;
+    DBuf<double> chunk_area;
     // metrics
     bool metrics_ready = false;
     int S = 0, q = 0, nb = 0, band_cap = 6144;  // scratch: S * band_cap band faces per source
@@ -339,6 +345,11 @@ struct fcw_ctx {
         md.max_iter = max_iter;
         md.pool_cap = S * band_cap;
         md.band = w.band.p;
+        md.nch = nch;
+        md.ch_lo = nullptr;  // set by the wiring path after chunk_stats
+        md.ch_hi = nullptr;
+        md.part = nullptr;
+        md.chunk_area = chunk_area.p;
         return md;
     }
 
@@ -556,6 +567,43 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
                                   C->g22.p);
         fcw::launch_assemble(st, n, nnz, C->cptr.p, C->contrib.p, C->area.p, C->gb.p, C->vf_ptr.p, C->vf.p,
                              C->lval.p, C->mass.p);
+        {
+            // 256-face chunks of the face order: their vertices, each vertex owned by the first
+            // chunk that holds it (a vertex in no face by the last chunk), owned vertices first
+            constexpr int kChunk = 256;
+            const int nch = (m + kChunk - 1) / kChunk;
+            std::vector<int> owner(n, -1), mark(n, -1);
+            for (int f = 0; f < m; ++f)
+                for (int a = 0; a < 3; ++a) {
+                    const int p = S.faces[3 * (size_t)f + a];
+                    if (owner[p] < 0) owner[p] = f / kChunk;
+                }
+            std::vector<std::vector<int>> owned(std::max(1, nch));
+            for (int p = 0; p < n; ++p) owned[owner[p] >= 0 ? owner[p] : std::max(0, nch - 1)].push_back(p);
+            std::vector<int> vptr(nch + 1, 0), verts, nown(std::max(1, nch), 0), other;
+            for (int c = 0; c < nch; ++c) {
+                other.clear();
+                for (int p : owned[c]) mark[p] = c;
+                for (int f = c * kChunk; f < std::min(m, (c + 1) * kChunk); ++f)
+                    for (int a = 0; a < 3; ++a) {
+                        const int p = S.faces[3 * (size_t)f + a];
+                        if (mark[p] == c) continue;
+                        mark[p] = c;
+                        other.push_back(p);
+                    }
+                std::sort(other.begin(), other.end());
+                nown[c] = (int)owned[c].size();
+                verts.insert(verts.end(), owned[c].begin(), owned[c].end());
+                verts.insert(verts.end(), other.begin(), other.end());
+                vptr[c + 1] = (int)verts.size();
+            }
+            C->nch = nch;
+            C->cvptr.upload(vptr.data(), vptr.size(), tb);
+            C->cverts.upload(verts.data(), std::max<size_t>(1, verts.size()), tb);
+            C->cnown.upload(nown.data(), std::max<size_t>(1, nown.size()), tb);
+            C->chunk_area.alloc(std::max(1, nch), tb);
+            fcw::launch_chunk_area(st, m, C->larea.p, C->chunk_area.p);
+        }
         HIPCHK(hipGetLastError());
         std::vector<double> lval(nnz), mass(n);
         HIPCHK(hipStreamSynchronize(st));
@@ -702,7 +750,15 @@ int fcw_set_metrics(fcw_ctx* C, const double* targets, int S, const double* frac
         C->cap = cap_fraction;
         C->rtol = rtol;
         C->max_iter = max_iter;
-        for (auto& w : C->ws) w->band.alloc((size_t)C->ldk * S * 5 * C->band_cap, tb);
+        for (auto& w : C->ws) {
+            w->band.alloc((size_t)C->ldk * S * 5 * C->band_cap, tb);
+            w->cpart.alloc((size_t)C->ldk * std::max(1, C->nch) * fcw::kChunkStats, tb);
+            w->ch_lo.alloc((size_t)C->ldk * std::max(1, C->nch), tb);
+            w->ch_hi.alloc((size_t)C->ldk * std::max(1, C->nch), tb);
+        }
+        std::vector<int> flag(C->n, 0);
+        for (int i = 0; i < nb; ++i) flag[bpos[i]] = 1;
+        C->isb.upload(flag.data(), flag.size(), tb);
         C->o_cap = 0;
         C->metrics_ready = true;
     });
@@ -734,8 +790,14 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
                 HIPCHK(hipEventCreate(&mb));
                 HIPCHK(hipEventRecord(ma, w.stream));
             }
-            fcw::launch_metrics(w.stream, k, mdv, C->metrics_dev(w), w.Dt.p, w.src_pos.p, C->outputs_at((size_t)b0),
-                                true);
+            fcw::launch_chunk_stats(w.stream, k, C->ldk, C->nch, w.Z.p, w.src_pos.p, C->V.p, C->vw.p, C->isb.p,
+                                    C->cvptr.p, C->cverts.p, C->cnown.p, C->eps, C->health_h, w.ch_lo.p, w.ch_hi.p,
+                                    w.cpart.p);
+            fcw::MetricsDev md = C->metrics_dev(w);
+            md.ch_lo = w.ch_lo.p;
+            md.ch_hi = w.ch_hi.p;
+            md.part = w.cpart.p;
+            fcw::launch_metrics(w.stream, k, mdv, md, w.Dt.p, w.src_pos.p, C->outputs_at((size_t)b0), true);
             C->ev_end(w, e);
             HIPCHK(hipGetLastError());
             if (C->profiling) {

@@ -68,7 +68,19 @@ struct MetricsDev {
     int max_iter;
     int pool_cap;           // band faces per source held in the scratch (all bands together)
     double* band;           // k x 5 x pool_cap (sorted a, b, c, area, |grad d| per band face)
+    // Far-field summaries from chunk_stats (null: the metric kernel makes its own vertex pass and
+    // chunk bounds): per source and 256-face chunk, the distance range of the chunk's vertices
+    // and kChunkStats partial sums over the vertices the chunk owns.
+    int nch;
+    const float* ch_lo;        // k x nch, rounded down
+    const float* ch_hi;        // k x nch, rounded up
+    const double* part;        // k x kChunkStats x nch
+    const double* chunk_area;  // nch: area of the chunk's faces
 };
+
+// Partial record of chunk_stats: sum d (d > eps), sum d*w, sum w, max finite d, min finite d on the
+// boundary, then as exact counts: d < -h, chord violations, d > eps, non-finite, finite boundary.
+constexpr int kChunkStats = 10;
 
 struct MetricsOut {
     double* scal;      // k x kSlotCount
@@ -100,6 +112,10 @@ void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const doubl
 void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
                             double* Dt);
 void launch_gather_rows(hipStream_t st, int n, int k, const double* Dt, const int* iperm, double* out);
+void launch_chunk_area(hipStream_t st, int m, const double* larea, double* chunk_area);
+void launch_chunk_stats(hipStream_t st, int k, int ldk, int nch, const double* Z, const int* src_pos, const double* V,
+                        const double* vw, const int* isb, const int* cvptr, const int* cverts, const int* cnown,
+                        double eps, double h, float* ch_lo, float* ch_hi, double* part);
 void launch_metrics(hipStream_t st, int k, const MeshDev& mesh, const MetricsDev& md, const double* Dt,
                     const int* src_pos, MetricsOut out, bool health_screen);
 

@@ -119,13 +119,16 @@ struct FaceBatch {
     double d0[UNR], d1[UNR], d2[UNR], A[UNR];
 };
 
-// Lane takes faces base + UNR*lane + u (consecutive faces per lane, 256 consecutive per wave).
+// A wave-step covers the 256 consecutive faces from base. CONSEC: lane takes base + UNR*lane + u
+// (runs of consecutive faces per lane: neighbouring faces share histogram bins, few collisions
+// across lanes); otherwise base + 64 u + lane (each step u spans a 64-face patch: few bands).
+template <bool CONSEC>
 __device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ larea,
                                   const double* __restrict__ d, int m, int base, int lane, FaceBatch& fb) {
     int i0[UNR], i1[UNR], i2[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        const int f = base + UNR * lane + u;
+        const int f = CONSEC ? base + UNR * lane + u : base + kWave * u + lane;
         fb.f[u] = f;
         const int fs = f < m ? f : m - 1;  // an in-bounds face; the caller ignores f >= m
         i0[u] = faces[3 * fs];
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __shared__ float ch_lo[kMaxChunks], ch_hi[kMaxChunks];  // per 256-face chunk: min dmin (rounded
                                                              // down), max dmax (rounded up)
     __shared__ int s_nbins, s_ovf;
-    __shared__ double s_skip_lo, s_skip_hi;
+    __shared__ double s_skip_lo, s_skip_hi, s_cut;
     __shared__ double s_wb;
     __shared__ double s_scal[kSlotCount];
 
@@ -323,8 +326,27 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const double xs = mesh.V[3 * sp], ys = mesh.V[3 * sp + 1], zs = mesh.V[3 * sp + 2];
     double v4[4] = {0.0, 0.0, 0.0, 0.0};  // sum d, sum d*w, sum w, max finite d
     int i4[4] = {0, 0, 0, 0};             // d < -h, chord violations, valid, non-finite
+    double b4[4] = {INFINITY, 0.0, 0.0, 0.0};
+    int bi4[4] = {0, 0, 0, 0};
     const double h = md.health_h;
-    for (int p0 = tid; p0 < n; p0 += NT * UNR) {
+    if (md.part) {
+        // the vertex pass was made for the whole block by chunk_stats: add up its chunk records
+        const double* pr = md.part + (int64_t)j * md.nch * kChunkStats;  // kChunkStats arrays of nch
+        const int64_t st = md.nch;
+        for (int c = tid; c < md.nch; c += NT) {
+            v4[0] += pr[c];
+            v4[1] += pr[st + c];
+            v4[2] += pr[2 * st + c];
+            v4[3] = fmax(v4[3], pr[3 * st + c]);
+            b4[0] = fmin(b4[0], pr[4 * st + c]);
+            i4[0] += (int)pr[5 * st + c];
+            i4[1] += (int)pr[6 * st + c];
+            i4[2] += (int)pr[7 * st + c];
+            i4[3] += (int)pr[8 * st + c];
+            bi4[0] += (int)pr[9 * st + c];
+        }
+    }
+    for (int p0 = md.part ? n : tid; p0 < n; p0 += NT * UNR) {
         double dv[UNR], vx[UNR], vy[UNR], vz[UNR], vw[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
@@ -360,9 +382,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     }
     block_reduce(red, v4, i4, 3, -1);
     const double dglob = v4[3];
-    double b4[4] = {INFINITY, 0.0, 0.0, 0.0};
-    int bi4[4] = {0, 0, 0, 0};
-    for (int i = tid; i < md.nb; i += NT) {
+    for (int i = md.part ? md.nb : tid; i < md.nb; i += NT) {
         const double dv = d[md.boundary[i]];
         if (isfinite(dv)) {
             b4[0] = fmin(b4[0], dv);
@@ -412,9 +432,9 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
 
     // ---- phase 1: dmax / dmin area and count histograms ------------------------------------
     if (tid == 0) {
-        // Bins only steer the work, never the result: a quarter of the reference bucket width,
-        // widened when the field is too long for the LDS histogram.
-        double wb = 0.25 * md.width;
+        // Bins only steer the work, never the result: an eighth of the reference bucket width
+        // (narrow brackets, small bands), widened when the field is too long for the histogram.
+        double wb = 0.125 * md.width;
         if (dglob / wb + 2.0 > (double)kMaxBins) wb = dglob / (double)(kMaxBins - 3);
         int nbins = (int)(dglob / wb) + 2;
         if (nbins > kMaxBins) nbins = kMaxBins;
@@ -431,10 +451,57 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
     const double wb = s_wb;
     const int nbins = s_nbins;
-    const bool chunked = m <= kMaxChunks * kWave * UNR;
+    // Chunk distance ranges: from chunk_stats when given (global), else recorded below (LDS).
+    const bool have_ch = md.ch_lo != nullptr;
+    const float* gch_lo = have_ch ? md.ch_lo + (int64_t)j * md.nch : nullptr;
+    const float* gch_hi = have_ch ? md.ch_hi + (int64_t)j * md.nch : nullptr;
+    const bool chunked = have_ch || m <= kMaxChunks * kWave * UNR;
+    // With chunk ranges and a finite field, only faces that start below `cut` are histogrammed:
+    // the chunks entirely below bin K already hold the largest target's area, so every radius,
+    // bracket, band and band capacity lies below bin K + 3 (eight bins of slack are kept).
+    if (tid == 0) s_cut = INFINITY;
+    if (have_ch && i4[3] == 0) {
+        double need = 0.0;
+        for (int s = S - 1; s >= 0; --s)
+            if (md.targets[s] == md.targets[s]) {
+                need = md.targets[s] * (1.0 + 1e-9);
+                break;
+            }
+        for (int c = tid; c < md.nch; c += NT) {
+            const double hi = (double)gch_hi[c];
+            if (hi > -INFINITY) atomicAdd(&Hmin[bin_of(hi, wb, nbins)], md.chunk_area[c]);
+        }
+        __syncthreads();
+        if (wid == 0) {
+            const int per = (nbins + kWave - 1) / kWave;
+            const int b0 = lane * per, b1 = min(nbins, b0 + per);
+            double loc = 0.0;
+            for (int b = b0; b < b1; ++b) loc += Hmin[b];
+            double inc = loc;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const double y = __shfl_up(inc, o, kWave);
+                if (lane >= o) inc += y;
+            }
+            double run = inc - loc;  // sum over bins < b0
+            int kfound = INT_MAX;     // first K with sum over bins < K >= need
+            if (run >= need) kfound = b0;
+            for (int b = b0; b < b1 && kfound == INT_MAX; ++b) {
+                run += Hmin[b];
+                if (run >= need) kfound = b + 1;
+            }
+            kfound = wave_min_i(kfound);
+            if (lane == 0 && kfound != INT_MAX) s_cut = (double)(kfound + 8) * wb;
+        }
+        __syncthreads();
+        for (int i = tid; i <= kMaxBins; i += NT) Hmin[i] = 0.0;
+        __syncthreads();
+    }
+    __syncthreads();
+    const double cut = s_cut;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+        if (have_ch && (double)gch_lo[base / (kWave * UNR)] >= cut) continue;  // starts beyond every band
         FaceBatch fb;
-        load_faces(faces, mesh.larea, d, m, base, lane, fb);
+        load_faces<true>(faces, mesh.larea, d, m, base, lane, fb);
         double cmin = INFINITY, cmax = -INFINITY;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
@@ -451,7 +518,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                 cmax = fmax(cmax, hi);
             }
         }
-        if (chunked) {
+        if (!have_ch && chunked) {
             // float bounds rounded outwards, reduced over the wave
             float flo = __double2float_rd(cmin), fhi = __double2float_ru(cmax);
             for (int o = kWave / 2; o > 0; o >>= 1) {
@@ -651,10 +718,12 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         if (chunked) {
             const int c = base / (kWave * UNR);
-            if ((double)ch_hi[c] < skip_lo || (double)ch_lo[c] >= skip_hi) continue;  // meets no band
+            const double clo = have_ch ? (double)gch_lo[c] : (double)ch_lo[c];
+            const double chi = have_ch ? (double)gch_hi[c] : (double)ch_hi[c];
+            if (chi < skip_lo || clo >= skip_hi) continue;  // meets no band
         }
         FaceBatch fb;
-        load_faces(faces, mesh.larea, d, m, base, lane, fb);
+        load_faces<false>(faces, mesh.larea, d, m, base, lane, fb);
         int ia[UNR], ib[UNR];
         double sa_[UNR], sb_[UNR], sc_[UNR];
         bool anyf = false;
@@ -914,7 +983,137 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     }
 }
 
+// Area of every 256-face chunk (the unit of the distance-range records), summed in face order.
+__global__ void chunk_area_kernel(int m, const double* __restrict__ larea, double* __restrict__ chunk_area) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f0 = c * kWave * UNR;
+    if (f0 >= m) return;
+    double a = 0.0;
+    for (int f = f0; f < min(m, f0 + kWave * UNR); ++f) a += larea[f];
+    chunk_area[c] = a;
+}
+
+// The far-field part of the per-source metrics for all fields of a block at once, vertex-major
+// (core_analysis.py:1558-1581, lean_geometry.py:343-357): one workgroup per (face chunk, 64
+// sources), one lane per source, the chunk's vertices shared by the four waves. For each source:
+// the distance range over the chunk's vertices (bounds every face of the chunk), and over the
+// vertices the chunk owns (each vertex is owned by exactly one chunk) the partial record of
+// kChunkStats values. d = phi[p][j] - phi[src_j][j], as the shifted fields.
+__global__ __launch_bounds__(256) void chunk_stats_kernel(int ldk, int k, int nch, const double* __restrict__ Z,
+                                                          const int* __restrict__ src_pos, const double* __restrict__ V,
+                                                          const double* __restrict__ vw, const int* __restrict__ isb,
+                                                          const int* __restrict__ cvptr, const int* __restrict__ cverts,
+                                                          const int* __restrict__ cnown, double eps, double h,
+                                                          float* __restrict__ ch_lo, float* __restrict__ ch_hi,
+                                                          double* __restrict__ part) {
+    constexpr int U = 4;
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int jl = blockIdx.y * kWave + lane;
+    const int jc = min(jl, k - 1);
+    const int sp = src_pos[jc];
+    const double base = Z[(int64_t)sp * ldk + jc];
+    const double xs = V[3 * sp], ys = V[3 * sp + 1], zs = V[3 * sp + 2];
+    double lo = INFINITY, hi = -INFINITY, sd = 0.0, sdw = 0.0, sw = 0.0, dmax = 0.0, bmin = INFINITY;
+    int nneg = 0, nviol = 0, nval = 0, nnf = 0, nbf = 0;
+    const int v0 = cvptr[c], v1 = cvptr[c + 1], vown = v0 + cnown[c];
+    for (int vb = v0 + w * U; vb < v1; vb += 4 * U) {
+        int p[U];
+        double z[U], px[U], py[U], pz[U], pw[U];
+        int pb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) p[u] = cverts[min(vb + u, v1 - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            z[u] = Z[(int64_t)p[u] * ldk + jc];
+            px[u] = V[3 * p[u]];
+            py[u] = V[3 * p[u] + 1];
+            pz[u] = V[3 * p[u] + 2];
+            pw[u] = vw[p[u]];
+            pb[u] = isb[p[u]];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int v = vb + u;  // wave-uniform
+            if (v >= v1) break;
+            const double x = z[u] - base;
+            const bool fin = isfinite(x);
+            if (fin) {
+                lo = fmin(lo, x);
+                hi = fmax(hi, x);
+            }
+            if (v >= vown) continue;
+            if (!fin) {
+                nnf += 1;
+                continue;
+            }
+            const double dx = px[u] - xs, dy = py[u] - ys, dz = pz[u] - zs;
+            const double chord = sqrt(dx * dx + dy * dy + dz * dz);
+            if (x < -h) nneg += 1;
+            if (x < 0.9 * chord - h) nviol += 1;
+            if (x > eps) {
+                nval += 1;
+                sd += x;
+                sdw += x * pw[u];
+                sw += pw[u];
+            }
+            dmax = fmax(dmax, x);
+            if (pb[u]) {
+                bmin = fmin(bmin, x);
+                nbf += 1;
+            }
+        }
+    }
+    // combine the four waves in wave order
+    __shared__ double red[4][kWave][kChunkStats + 2];
+    double* r = red[w][lane];
+    r[0] = sd;
+    r[1] = sdw;
+    r[2] = sw;
+    r[3] = dmax;
+    r[4] = bmin;
+    r[5] = nneg;
+    r[6] = nviol;
+    r[7] = nval;
+    r[8] = nnf;
+    r[9] = nbf;
+    r[10] = lo;
+    r[11] = hi;
+    __syncthreads();
+    if (w != 0 || jl >= k) return;
+    for (int ww = 1; ww < 4; ++ww) {
+        const double* o = red[ww][lane];
+        r[0] += o[0];
+        r[1] += o[1];
+        r[2] += o[2];
+        r[3] = fmax(r[3], o[3]);
+        r[4] = fmin(r[4], o[4]);
+        for (int q = 5; q < kChunkStats; ++q) r[q] += o[q];
+        r[10] = fmin(r[10], o[10]);
+        r[11] = fmax(r[11], o[11]);
+    }
+    double* out = part + (int64_t)jl * nch * kChunkStats + c;  // per source: kChunkStats arrays of nch
+    for (int q = 0; q < kChunkStats; ++q) out[(int64_t)q * nch] = r[q];
+    ch_lo[(int64_t)jl * nch + c] = __double2float_rd(r[10]);
+    ch_hi[(int64_t)jl * nch + c] = __double2float_ru(r[11]);
+}
+
 }  // namespace
+
+void launch_chunk_area(hipStream_t st, int m, const double* larea, double* chunk_area) {
+    const int nch = (m + kWave * UNR - 1) / (kWave * UNR);
+    if (nch <= 0) return;
+    chunk_area_kernel<<<dim3((nch + 255) / 256), 256, 0, st>>>(m, larea, chunk_area);
+}
+
+void launch_chunk_stats(hipStream_t st, int k, int ldk, int nch, const double* Z, const int* src_pos, const double* V,
+                        const double* vw, const int* isb, const int* cvptr, const int* cverts, const int* cnown,
+                        double eps, double h, float* ch_lo, float* ch_hi, double* part) {
+    if (k <= 0 || nch <= 0) return;
+    chunk_stats_kernel<<<dim3(nch, (k + kWave - 1) / kWave), 256, 0, st>>>(ldk, k, nch, Z, src_pos, V, vw, isb, cvptr,
+                                                                        cverts, cnown, eps, h, ch_lo, ch_hi, part);
+}
 
 void launch_metrics(hipStream_t st, int k, const MeshDev& mesh, const MetricsDev& md, const double* Dt,
                     const int* src_pos, MetricsOut out, bool health_screen) {
