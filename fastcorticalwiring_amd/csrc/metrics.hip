@@ -994,11 +994,14 @@ __global__ void chunk_area_kernel(int m, const double* __restrict__ larea, doubl
 }
 
 // The far-field part of the per-source metrics for all fields of a block at once, vertex-major
-// (core_analysis.py:1558-1581, lean_geometry.py:343-357): one workgroup per (face chunk, 64
-// sources), one lane per source, the chunk's vertices shared by the four waves. For each source:
-// the distance range over the chunk's vertices (bounds every face of the chunk), and over the
-// vertices the chunk owns (each vertex is owned by exactly one chunk) the partial record of
-// kChunkStats values. d = phi[p][j] - phi[src_j][j], as the shifted fields.
+// (core_analysis.py:1558-1581, lean_geometry.py:343-357): one workgroup per face chunk and 256
+// sources, one lane per source, each wave walking all of the chunk's vertices (their indices
+// staged in LDS) for its 64 sources, 8 vertices per step with every load of the step in flight
+// together. For each source: the distance range over the chunk's vertices (bounds every face of
+// the chunk), and over the vertices the chunk owns (each vertex is owned by exactly one chunk)
+// the partial record of kChunkStats values. d = phi[p][j] - phi[src_j][j], as the shifted fields.
+constexpr int kStatsVerts = 1024;  // chunk vertices staged in LDS per page (face vertices: at most 768)
+
 __global__ __launch_bounds__(256) void chunk_stats_kernel(int ldk, int k, int nch, const double* __restrict__ Z,
                                                           const int* __restrict__ src_pos, const double* __restrict__ V,
                                                           const double* __restrict__ vw, const int* __restrict__ isb,
@@ -1006,97 +1009,80 @@ __global__ __launch_bounds__(256) void chunk_stats_kernel(int ldk, int k, int nc
                                                           const int* __restrict__ cnown, double eps, double h,
                                                           float* __restrict__ ch_lo, float* __restrict__ ch_hi,
                                                           double* __restrict__ part) {
-    constexpr int U = 4;
+    constexpr int U = 8;
+    __shared__ int verts[kStatsVerts];
     const int c = blockIdx.x;
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    const int jl = blockIdx.y * kWave + lane;
+    const int v0 = cvptr[c], ntot = cvptr[c + 1] - v0, nown_all = cnown[c];
+    const int jl = blockIdx.y * 256 + threadIdx.x;
+    const bool wave_on = blockIdx.y * 256 + (int)(threadIdx.x & ~(kWave - 1)) < k;  // wave holds a source
     const int jc = min(jl, k - 1);
     const int sp = src_pos[jc];
     const double base = Z[(int64_t)sp * ldk + jc];
     const double xs = V[3 * sp], ys = V[3 * sp + 1], zs = V[3 * sp + 2];
     double lo = INFINITY, hi = -INFINITY, sd = 0.0, sdw = 0.0, sw = 0.0, dmax = 0.0, bmin = INFINITY;
     int nneg = 0, nviol = 0, nval = 0, nnf = 0, nbf = 0;
-    const int v0 = cvptr[c], v1 = cvptr[c + 1], vown = v0 + cnown[c];
-    for (int vb = v0 + w * U; vb < v1; vb += 4 * U) {
-        int p[U];
-        double z[U], px[U], py[U], pz[U], pw[U];
-        int pb[U];
+    for (int pg = 0; pg < ntot; pg += kStatsVerts) {  // one page unless the chunk owns faceless vertices
+        const int nv = min(ntot - pg, kStatsVerts), nown = nown_all - pg;
+        __syncthreads();
+        for (int i = threadIdx.x; i < nv; i += blockDim.x) verts[i] = cverts[v0 + pg + i];
+        __syncthreads();
+        for (int vb = 0; wave_on && vb < nv; vb += U) {
+            int p[U];
+            double z[U], px[U], py[U], pz[U], pw[U];
+            int pb[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) p[u] = cverts[min(vb + u, v1 - 1)];
+            for (int u = 0; u < U; ++u) p[u] = verts[min(vb + u, nv - 1)];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            z[u] = Z[(int64_t)p[u] * ldk + jc];
-            px[u] = V[3 * p[u]];
-            py[u] = V[3 * p[u] + 1];
-            pz[u] = V[3 * p[u] + 2];
-            pw[u] = vw[p[u]];
-            pb[u] = isb[p[u]];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+            for (int u = 0; u < U; ++u) {
+                z[u] = Z[(int64_t)p[u] * ldk + jc];
+                px[u] = V[3 * p[u]];
+                py[u] = V[3 * p[u] + 1];
+                pz[u] = V[3 * p[u] + 2];
+                pw[u] = vw[p[u]];
+                pb[u] = isb[p[u]];
+            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int v = vb + u;  // wave-uniform
-            if (v >= v1) break;
-            const double x = z[u] - base;
-            const bool fin = isfinite(x);
-            if (fin) {
-                lo = fmin(lo, x);
-                hi = fmax(hi, x);
-            }
-            if (v >= vown) continue;
-            if (!fin) {
-                nnf += 1;
-                continue;
-            }
-            const double dx = px[u] - xs, dy = py[u] - ys, dz = pz[u] - zs;
-            const double chord = sqrt(dx * dx + dy * dy + dz * dz);
-            if (x < -h) nneg += 1;
-            if (x < 0.9 * chord - h) nviol += 1;
-            if (x > eps) {
-                nval += 1;
-                sd += x;
-                sdw += x * pw[u];
-                sw += pw[u];
-            }
-            dmax = fmax(dmax, x);
-            if (pb[u]) {
-                bmin = fmin(bmin, x);
-                nbf += 1;
+            for (int u = 0; u < U; ++u) {
+                const int v = vb + u;  // uniform
+                const double x = z[u] - base;
+                const bool fin = isfinite(x);
+                const bool in = v < nv;
+                if (in && fin) {
+                    lo = fmin(lo, x);
+                    hi = fmax(hi, x);
+                }
+                if (!in || v >= nown) continue;
+                if (!fin) {
+                    nnf += 1;
+                    continue;
+                }
+                const double dx = px[u] - xs, dy = py[u] - ys, dz = pz[u] - zs;
+                const double chord = sqrt(dx * dx + dy * dy + dz * dz);
+                if (x < -h) nneg += 1;
+                if (x < 0.9 * chord - h) nviol += 1;
+                if (x > eps) {
+                    nval += 1;
+                    sd += x;
+                    sdw += x * pw[u];
+                    sw += pw[u];
+                }
+                dmax = fmax(dmax, x);
+                if (pb[u]) {
+                    bmin = fmin(bmin, x);
+                    nbf += 1;
+                }
             }
         }
     }
-    // combine the four waves in wave order
-    __shared__ double red[4][kWave][kChunkStats + 2];
-    double* r = red[w][lane];
-    r[0] = sd;
-    r[1] = sdw;
-    r[2] = sw;
-    r[3] = dmax;
-    r[4] = bmin;
-    r[5] = nneg;
-    r[6] = nviol;
-    r[7] = nval;
-    r[8] = nnf;
-    r[9] = nbf;
-    r[10] = lo;
-    r[11] = hi;
-    __syncthreads();
-    if (w != 0 || jl >= k) return;
-    for (int ww = 1; ww < 4; ++ww) {
-        const double* o = red[ww][lane];
-        r[0] += o[0];
-        r[1] += o[1];
-        r[2] += o[2];
-        r[3] = fmax(r[3], o[3]);
-        r[4] = fmin(r[4], o[4]);
-        for (int q = 5; q < kChunkStats; ++q) r[q] += o[q];
-        r[10] = fmin(r[10], o[10]);
-        r[11] = fmax(r[11], o[11]);
-    }
+    if (jl >= k) return;
+    const double r[kChunkStats] = {sd, sdw, sw, dmax, bmin, (double)nneg, (double)nviol, (double)nval, (double)nnf,
+                                   (double)nbf};
     double* out = part + (int64_t)jl * nch * kChunkStats + c;  // per source: kChunkStats arrays of nch
+#pragma unroll
     for (int q = 0; q < kChunkStats; ++q) out[(int64_t)q * nch] = r[q];
-    ch_lo[(int64_t)jl * nch + c] = __double2float_rd(r[10]);
-    ch_hi[(int64_t)jl * nch + c] = __double2float_ru(r[11]);
+    ch_lo[(int64_t)jl * nch + c] = __double2float_rd(lo);
+    ch_hi[(int64_t)jl * nch + c] = __double2float_ru(hi);
 }
 
 }  // namespace
@@ -1111,8 +1097,8 @@ void launch_chunk_stats(hipStream_t st, int k, int ldk, int nch, const double* Z
                         const double* vw, const int* isb, const int* cvptr, const int* cverts, const int* cnown,
                         double eps, double h, float* ch_lo, float* ch_hi, double* part) {
     if (k <= 0 || nch <= 0) return;
-    chunk_stats_kernel<<<dim3(nch, (k + kWave - 1) / kWave), 256, 0, st>>>(ldk, k, nch, Z, src_pos, V, vw, isb, cvptr,
-                                                                        cverts, cnown, eps, h, ch_lo, ch_hi, part);
+    chunk_stats_kernel<<<dim3(nch, (k + 255) / 256), 256, 0, st>>>(ldk, k, nch, Z, src_pos, V, vw, isb, cvptr, cverts,
+                                                                   cnown, eps, h, ch_lo, ch_hi, part);
 }
 
 void launch_metrics(hipStream_t st, int k, const MeshDev& mesh, const MetricsDev& md, const double* Dt,
