@@ -78,19 +78,30 @@ def cpu_worker(workload, n_sources, out_path, seed=0):
     src = rng.choice(mesh.n, size=min(n_sources, mesh.n), replace=False)
     T = prep["targets"]
     fr = prep["fracs"]
+    keep = dict(D=[], radii=[], perim=[], msd_u=[], msd_w=[], flag=[])
     t0 = time.perf_counter()
     for sidx in src.tolist():  # one source at a time, the reference loop's shape (core_analysis.py:1550-1619)
         d = H.compute_distance(sidx)
         neg, viol, bad = og.field_health(d, mesh.V, sidx, mesh.health_h)
+        keep["D"].append(d)
+        keep["flag"].append(bad)
         if bad:
+            for key in ("radii", "perim"):
+                keep[key].append(np.full(T.size, np.nan))
+            keep["msd_u"].append(np.nan)
+            keep["msd_w"].append(np.nan)
             continue
         if mesh.boundary.size:
             bd = d[mesh.boundary]
             _ = np.min(bd[np.isfinite(bd)])
         valid = (d > mesh.eps) & np.isfinite(d)
-        _ = np.mean(d[valid]), float((d[valid] * mesh.vertex_areas[valid]).sum() / mesh.vertex_areas[valid].sum())
-        og.per_source_geometry(d, mesh.F, mesh.lean_area, mesh.g11, mesh.g12, mesh.g22, mesh.width, T, fr)
+        keep["msd_u"].append(np.mean(d[valid]))
+        keep["msd_w"].append(float((d[valid] * mesh.vertex_areas[valid]).sum() / mesh.vertex_areas[valid].sum()))
+        g = og.per_source_geometry(d, mesh.F, mesh.lean_area, mesh.g11, mesh.g12, mesh.g22, mesh.width, T, fr)
+        keep["radii"].append(g["radii"][0])
+        keep["perim"].append(g["perim"][0])
     dt = time.perf_counter() - t0
+    np.savez(out_path + ".npz", src=src, health_h=mesh.health_h, **{k: np.asarray(v) for k, v in keep.items()})
     res = dict(value=src.size / dt, unit="sources/s", cores=1, kind="port",
                sample=f"{src.size} seeded sources of the same mesh, one at a time: SuperLU (MMD on A^T+A, symmetric mode) FP64 "
                       f"heat + Poisson solves, C fused divergence, numpy health/MSD, C lean geometry; "
@@ -103,7 +114,7 @@ def cpu_worker(workload, n_sources, out_path, seed=0):
 # --------------------------------------------------------------------------------------------
 # GPU measurement
 # --------------------------------------------------------------------------------------------
-def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4):
+def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4, check_src=None):
     from fastcorticalwiring_amd.engine import HeatContext
 
     t0 = time.perf_counter()
@@ -126,10 +137,18 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4):
     wall = time.perf_counter() - t0
     flags = int(np.sum(res["scalars"][:, 0] > 0))
     ovf = int(np.sum(res["scalars"][:, 8] > 0))
+    # outputs kept for the accuracy report (not timed): all sources' metrics in `src` order, and
+    # the distance fields of the CPU leg's sample sources
+    outputs = dict(src=src, radii=res["radii"], perim=res["perimeters"], msd_u=res["scalars"][:, 4],
+                   msd_w=res["scalars"][:, 5])
+    if check_src is not None:
+        chk = ctx.wiring(check_src)
+        outputs["check"] = dict(src=check_src, D=ctx.distances(check_src), radii=chk["radii"],
+                                perim=chk["perimeters"], msd_u=chk["scalars"][:, 4], msd_w=chk["scalars"][:, 5])
     out = dict(precision=precision, sources=int(src.size), steps=int(steps), seconds=wall,
                sources_per_s=src.size / wall, ms_per_step=1e3 * wall / steps, setup_s=setup,
                hemisphere_latency_s=setup + wall * (n_src / src.size), flagged=flags, band_overflow=ovf,
-               info=ctx.info)
+               info=ctx.info, outputs=outputs)
     # per-kernel timing with HIP events on the context's stream
     ctx.set_profiling(True)
     ctx.wiring(np.resize(order, profile_blocks * k))
@@ -138,6 +157,29 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4):
     out["profile_blocks"] = profile_blocks
     ctx.close()
     return out
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    ok = np.isfinite(a) & np.isfinite(b)
+    if not ok.any():
+        return None
+    return float(np.max(np.abs(a[ok] - b[ok]) / np.maximum(np.abs(b[ok]), 1e-300)))
+
+
+def accuracy(out, ref, health_h=None):
+    """Max relative differences of one form's outputs against a reference (the CPU oracle's sample,
+    or the FP64 form over all sources); distances as compare_fastcw_lean.py:235-239 (relative where
+    the reference exceeds 5h, and max abs / max)."""
+    acc = {"msd_u_rel": _rel(out["msd_u"], ref["msd_u"]), "msd_w_rel": _rel(out["msd_w"], ref["msd_w"]),
+           "radius_rel": _rel(out["radii"], ref["radii"]), "perimeter_rel": _rel(out["perim"], ref["perim"])}
+    if "D" in out and "D" in ref:
+        D, R = np.asarray(out["D"]), np.asarray(ref["D"])
+        acc["distance_max_abs_over_max"] = float(np.max(np.abs(D - R)) / np.max(R))
+        far = R > 5.0 * health_h
+        acc["distance_rel_far"] = float(np.max(np.abs(D[far] - R[far]) / R[far]))
+    return acc
 
 
 def roofline(gpu, prep, block):
@@ -229,12 +271,15 @@ def main():
     if args.calibrate_bytes:
         _lib.check(_lib.lib().fcw_calibration_copy(int(args.calibrate_bytes), 0))
     forms = ["fp64", "fp32_ir"] if args.precision == "auto" else [args.precision]
+    # the CPU leg's sample sources (same seed and rule as cpu_worker), checked on the GPU as well
+    check_src = np.random.default_rng(0).choice(prep["V"].shape[0], size=min(args.cpu_sources, prep["V"].shape[0]),
+                                                replace=False)
     results = {}
     for form in forms:
         try:
             log(f"bench: timing {form}")
             results[form] = run_gpu(prep, form, args.block, args.steps, args.warmup, args.profile_blocks,
-                                    args.streams)
+                                    args.streams, check_src=check_src)
             r = results[form]
             log(f"bench: {form}: {r['sources_per_s']:.1f} sources/s ({r['ms_per_step']:.2f} ms/block), "
                 f"setup {r['setup_s']:.2f}s")
@@ -260,6 +305,7 @@ def main():
         log(f"bench: serving p50 {serving['p50_s']:.2f}s p95 {serving['p95_s']:.2f}s "
             f"p99 {serving['p99_s']:.2f}s, {serving['hemispheres_per_hour']:.0f} hemispheres/hour")
     cpu = None
+    oracle_out = None
     if cpu_proc is not None:
         log("bench: waiting for the CPU baseline leg")
         rc = cpu_proc.wait()
@@ -267,6 +313,22 @@ def main():
             with open(cpu_out) as f:
                 cpu = json.load(f)
             os.remove(cpu_out)
+        if rc == 0 and os.path.exists(cpu_out + ".npz"):
+            with np.load(cpu_out + ".npz") as z:
+                oracle_out = {k: z[k] for k in z.files}
+            os.remove(cpu_out + ".npz")
+    accuracy_report = {}
+    for form, r in results.items():
+        rep = {}
+        if oracle_out is not None and np.array_equal(oracle_out["src"], r["outputs"]["check"]["src"]):
+            rep["vs_oracle_sample"] = accuracy(r["outputs"]["check"], oracle_out, float(oracle_out["health_h"]))
+        if form != "fp64" and "fp64" in results:
+            a, b = r["outputs"], results["fp64"]["outputs"]
+            if np.array_equal(a["src"], b["src"]):
+                rep["vs_fp64_all_sources"] = accuracy(a, b)
+            rep["vs_fp64_sample"] = accuracy(r["outputs"]["check"], results["fp64"]["outputs"]["check"],
+                                             prep["health_h"])
+        accuracy_report[form] = rep
     roof = dict(roofs[dom])
     line = {
         "metric": METRIC,
@@ -300,6 +362,7 @@ def main():
                       "hemisphere_latency_s": v["hemisphere_latency_s"], "flagged": v["flagged"]}
                   for k, v in results.items()},
         "hemisphere_latency_s": best["hemisphere_latency_s"],
+        "accuracy": accuracy_report,
         "serving": serving,
         "other_kernels": {k: {kk: v[kk] for kk in ("achieved", "unit", "frac", "avg_launch_ms", "launches")}
                           for k, v in roofs.items() if k != dom},

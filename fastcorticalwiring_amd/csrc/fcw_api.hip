@@ -93,6 +93,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
     DBuf<double> cpart;              // chunk_stats records (k x nch x kChunkStats)
+    DBuf<float> X32, Y32, U32, Z32;  // refined solve (FCW_FP32_IR): FP32 right-hand side and sweeps
     DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
     DBuf<int> src_pos;
     std::vector<EventPair> ev_pool;
@@ -119,6 +120,7 @@ struct fcw_ctx {
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
     bool fwd_idx_lds = true, bwd_idx_lds = true;  // every node's sweep indices fit in LDS
     DBuf<double> panel_heat, panel_pois;
+    DBuf<float> panel_pois32;  // FCW_FP32_IR: the Poisson panels rounded to FP32
     DBuf<double> Dout;
     // per-call plans
     DBuf<fcw::LevelTile> plan_tiles;
@@ -212,6 +214,12 @@ struct fcw_ctx {
         w.Up.alloc(urows * ld, dev_bytes);
         w.Dt.alloc((size_t)n * ld, dev_bytes);
         w.src_pos.alloc(ld, dev_bytes);
+        if (opt.precision == FCW_FP32_IR) {
+            w.X32.alloc((size_t)n * ld, dev_bytes);
+            w.Y32.alloc((size_t)n * ld, dev_bytes);
+            w.Z32.alloc((size_t)n * ld, dev_bytes);
+            w.U32.alloc(urows * ld, dev_bytes);
+        }
         fcw::launch_fill(w.stream, w.Yh.p, (int64_t)n * ldk, 0.0);
         fcw::launch_fill(w.stream, w.Uh.p, (int64_t)urows * ldk, 0.0);
         HIPCHK(hipStreamSynchronize(w.stream));
@@ -249,6 +257,38 @@ struct fcw_ctx {
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
     }
 
+    // FP32 sweeps of one system: w.X32 -> w.Z32, the solution also stored / added into acc64.
+    void solve32(Workspace& w, const float* panel, double* acc64, int acc_mode) {
+        const int L = sym.nlevels;
+        for (int l = 0; l < L; ++l) {
+            EventPair* e = ev_begin(w, kFwdGemm);
+            fcw::launch_fwd_gemm(w.stream, fwd_tiles.p + fwd_tile_off[l], fwd_tile_off[l + 1] - fwd_tile_off[l], ldk,
+                                 node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, cown.p, ctail.p, panel,
+                                 w.X32.p, w.Y32.p, w.U32.p, fwd_idx_lds, l == 0);
+            ev_end(w, e);
+        }
+        for (int l = L - 1; l >= 0; --l) {
+            EventPair* e = ev_begin(w, kBwdGemm);
+            fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
+                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
+                                 w.Z32.p, bwd_idx_lds, acc64, acc_mode);
+            ev_end(w, e);
+        }
+    }
+
+    // Poisson system by refinement: an FP32 solve of the right-hand side, then ir_steps times the
+    // FP64 residual r = b - A x rounded to FP32 and an FP32 solve of A dx = r added to x (FP64).
+    void poisson_refined(Workspace& w) {
+        fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p, nullptr,
+                               w.X32.p);
+        solve32(w, panel_pois32.p, w.Z.p, 1);
+        for (int it = 0; it < opt.ir_steps; ++it) {
+            fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p,
+                                   w.Z.p, w.X32.p);
+            solve32(w, panel_pois32.p, w.Z.p, 2);
+        }
+    }
+
     // Distances for the k sources whose positions are in w.src_pos[0:k]; result in w.Dt (k x n).
     void distances(Workspace& w, int k, const BlockPlan* plan) {
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -264,7 +304,10 @@ struct fcw_ctx {
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
         fcw::launch_divergence(w.stream, mesh_dev(), ldk, w.Z.p, w.X.p);
         if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
-        solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p);
+        if (opt.precision == FCW_FP32_IR)
+            poisson_refined(w);
+        else
+            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p);
         fcw::launch_shift_transpose(w.stream, n, ldk, k, w.Z.p, w.src_pos.p, w.Dt.p);
         if (profiling) {
             HIPCHK(hipEventRecord(ev[3], w.stream));
@@ -484,7 +527,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->leaf_size = 64;
     opt->host_threads = 0;
     opt->device = 0;
-    opt->ir_steps = 2;
+    opt->ir_steps = 1;
     opt->streams = 4;
     opt->sparse_rhs = 1;
 }
@@ -512,7 +555,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         fcw_default_options(&opt);
         if (opt_in) opt = *opt_in;
         if (opt.precision != FCW_FP64 && opt.precision != FCW_FP32_IR) throw Fail("unknown precision");
-        if (opt.precision == FCW_FP32_IR) throw Fail("FCW_FP32_IR is not available in this build");
+        if (opt.ir_steps < 0 || opt.ir_steps > 8) throw Fail("ir_steps must be in 0..8");
         if (opt.block <= 0) opt.block = 256;
         if (opt.streams <= 0) opt.streams = 1;
         if (opt.streams > 8) throw Fail("streams must be in 1..8");
@@ -618,6 +661,10 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         double t3 = now_s();
         C->panel_heat.upload(ph.data(), ph.size(), tb);
         C->panel_pois.upload(pp.data(), pp.size(), tb);
+        if (opt.precision == FCW_FP32_IR) {
+            C->panel_pois32.alloc(pp.size(), tb);
+            fcw::launch_to_float(st, C->panel_pois.p, C->panel_pois32.p, (int64_t)pp.size());
+        }
         C->node_start.upload(S.node_start.data(), S.node_start.size(), tb);
         C->node_s.upload(S.node_s.data(), S.node_s.size(), tb);
         C->node_b.upload(S.node_b.data(), S.node_b.size(), tb);

@@ -108,6 +108,20 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds);
+// FP32 forms of the sweeps (refined solve): FP32 panels and right-hand sides, FP32 matrix-core
+// accumulation; the backward sweep also stores (acc_mode 1) or adds (2) its solution into acc64.
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
+                     const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,
+                     float* U, bool idx_lds, bool leaves);
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
+                     const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
+                     double* acc64, int acc_mode);
+// r = b - (L + shift M) x in FP64 (x null: r = b), rounded to FP32 (the refined solve's RHS).
+void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
+                       const double* mass, double shift, const double* b, const double* x, float* r);
+void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count);
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out);
 void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
                             double* Dt);

@@ -229,6 +229,46 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Refined (FP32 + FP64 residual) solve support
+// ---------------------------------------------------------------------------
+// r[p] = b[p] - sum_q A[p][q] x[q] with A = L + shift M (CSR of L in positions, lumped M), in
+// FP64 and in CSR order, then rounded to FP32. One wave per row, one lane per RHS column.
+__global__ __launch_bounds__(256) void residual32_kernel(int n, int ldk, const int* __restrict__ rowptr,
+                                                         const int* __restrict__ colind, const double* __restrict__ lval,
+                                                         const double* __restrict__ mass, double shift,
+                                                         const double* __restrict__ b, const double* __restrict__ x,
+                                                         float* __restrict__ r) {
+    const int col = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    double v = b[(int64_t)row * ldk + col];
+    if (x) {
+        const int e0 = rowptr[row], e1 = rowptr[row + 1];
+        double ax = 0.0;
+        constexpr int kC = 8;
+        for (int e = e0; e < e1; e += kC) {
+            double xv[kC], a[kC];
+#pragma unroll
+            for (int q = 0; q < kC; ++q) {
+                const int ec = min(e + q, e1 - 1);
+                const int cq = colind[ec];
+                a[q] = lval[ec] + (cq == row ? shift * mass[row] : 0.0);
+                xv[q] = x[(int64_t)cq * ldk + col];
+            }
+#pragma unroll
+            for (int q = 0; q < kC; ++q) ax += (e + q < e1) ? a[q] * xv[q] : 0.0;
+        }
+        v -= ax;
+    }
+    r[(int64_t)row * ldk + col] = (float)v;
+}
+
+__global__ void to_float_kernel(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
 // Dt[j][p] = phi[p][j] - phi[src_j][j]   (source-major distance fields)
 __global__ void shift_transpose_kernel(int n, int ldk, int k, const double* __restrict__ phi,
                                        const int* __restrict__ src_pos, double* __restrict__ Dt) {
@@ -303,6 +343,16 @@ void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const doubl
     const int padded = (ngroups + kXcds - 1) / kXcds * kXcds;  // whole rounds of the XCD mapping
     divergence_kernel<<<dim3(padded * (ldk / 64)), 256, 0, st>>>(mesh.n, ldk, ngroups, mesh.vf_ptr, mesh.vf,
                                                                  mesh.faces, mesh.area, mesh.gb, u, out);
+}
+
+void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
+                       const double* mass, double shift, const double* b, const double* x, float* r) {
+    residual32_kernel<<<dim3((n + 3) / 4, ldk / 64), 256, 0, st>>>(n, ldk, rowptr, colind, lval, mass, shift, b, x, r);
+}
+
+void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count) {
+    if (count <= 0) return;
+    to_float_kernel<<<grid1(count, 256), 256, 0, st>>>(src, dst, count);
 }
 
 void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,

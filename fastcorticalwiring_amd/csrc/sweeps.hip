@@ -28,6 +28,24 @@ namespace fcw {
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// One 16 x 16 x 4 matrix-core step per value type, and where it leaves row r of lane group q
+// (checked on the device with one-wave probes): FP64 D rows q + 4 r, FP32 D rows 4 q + r.
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+    typedef dbl4 acc_t;
+    static __device__ acc_t step(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ int row(int q, int r) { return q + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+    typedef flt4 acc_t;
+    static __device__ acc_t step(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ int row(int q, int r) { return 4 * q + r; }
+};
 
 constexpr int kThreads = 256;
 constexpr int kPad = 2;
@@ -38,17 +56,20 @@ static_assert(kTileM == 64 && kTileN == 64 && kTileK == 16, "MFMA sweep tiling a
 
 enum class Sweep { Forward, Backward };
 
-// KIDS = false: every node of the launch is a leaf (no child update rows to gather).
-template <Sweep MODE, bool IDX_LDS, bool KIDS>
+// T: value type of panels, right-hand sides and solution (double, or float with FP32
+// accumulation for the refined solve). KIDS = false: every node of the launch is a leaf (no
+// child update rows to gather). Backward, acc_mode 1 / 2 also stores / adds each solution value
+// into the FP64 array acc64 (the iterate of the refined solve).
+template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
-    const int* __restrict__ ctail, const double* __restrict__ panel, const double* __restrict__ Bsrc,
-    const double* Zsrc, double* Out, double* U) {
+    const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
+    T* U, double* acc64, int acc_mode) {
     const LevelTile t = tiles[blockIdx.x];
     const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
-    const double* __restrict__ P = panel + node_poff[t.node];
+    const T* __restrict__ P = panel + node_poff[t.node];
     const int* __restrict__ B = (MODE == Sweep::Backward) ? bidx + node_boff[t.node] : nullptr;
     const int n0 = blockIdx.y * kTileN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -56,8 +77,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const int M = (MODE == Sweep::Forward) ? f : s;   // output rows of this node
     const int K = (MODE == Sweep::Forward) ? s : f;   // depth
 
-    __shared__ double As[kTileK][kTileM + kPad];
-    __shared__ double Bs[kTileK][kTileN + kPad];
+    __shared__ T As[kTileK][kTileM + kPad];
+    __shared__ T Bs[kTileK][kTileN + kPad];
     __shared__ int sidx[2 * kIdxCap];
     // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
     // contributor rows for each of the node's s own rows.
@@ -75,7 +96,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     // address. fetch() only issues the loads of the next step (and records which elements are
     // kept, subtracted or negated); stash() applies that after the current step's MFMAs, so no
     // arithmetic on a loaded value forces a wait before the matrix cores have their work.
-    double ra[kLoadsA], rb[kLoadsB], ru[kLoadsB], rv[kLoadsB];
+    T ra[kLoadsA], rb[kLoadsB], ru[kLoadsB], rv[kLoadsB];
     unsigned keep_a = 0, keep_b = 0, bit_u = 0, bit_v = 0;
     auto fetch = [&](int k0) {
         keep_a = keep_b = bit_u = bit_v = 0;
@@ -126,7 +147,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 // one load from either buffer, its base chosen by integer select (a pointer select
                 // is split by the compiler into two loads under branches)
                 const uintptr_t base = bnd ? reinterpret_cast<uintptr_t>(Zsrc) : reinterpret_cast<uintptr_t>(Bsrc);
-                rb[i] = reinterpret_cast<const double*>(base)[(int64_t)prow * ldk + c];
+                rb[i] = reinterpret_cast<const T*>(base)[(int64_t)prow * ldk + c];
             }
         }
     };
@@ -134,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
         for (int i = 0; i < kLoadsA; ++i) {
             const int e = tid + i * kThreads;
-            const double v = ((keep_a >> i) & 1u) ? ra[i] : 0.0;
+            const T v = ((keep_a >> i) & 1u) ? ra[i] : T(0);
             if (MODE == Sweep::Forward)
                 As[e / kTileM][e % kTileM] = v;
             else
@@ -143,22 +164,22 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
         for (int i = 0; i < kLoadsB; ++i) {
             const int e = tid + i * kThreads;
-            double v;
+            T v;
             if (MODE == Sweep::Forward && KIDS)
-                v = rb[i] - (((bit_u >> i) & 1u) ? ru[i] : 0.0) - (((bit_v >> i) & 1u) ? rv[i] : 0.0);
+                v = rb[i] - (((bit_u >> i) & 1u) ? ru[i] : T(0)) - (((bit_v >> i) & 1u) ? rv[i] : T(0));
             else if (MODE == Sweep::Forward)
                 v = rb[i];
             else
                 v = ((bit_u >> i) & 1u) ? -rb[i] : rb[i];
-            Bs[e / kTileN][e % kTileN] = ((keep_b >> i) & 1u) ? v : 0.0;
+            Bs[e / kTileN][e % kTileN] = ((keep_b >> i) & 1u) ? v : T(0);
         }
     };
 
-    dbl4 acc[2][2];
+    typename Mfma<T>::acc_t acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 2; ++j) acc[i][j] = typename Mfma<T>::acc_t{T(0), T(0), T(0), T(0)};
 
     const int lrow = lane & 15, lk = lane >> 4;
     if (K > 0) fetch(0);
@@ -169,14 +190,14 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         if (k0 + kTileK < K) fetch(k0 + kTileK);
 #pragma unroll
         for (int k4 = 0; k4 < kTileK; k4 += 4) {
-            double a0 = As[k4 + lk][wm0 + lrow];
-            double a1 = As[k4 + lk][wm0 + 16 + lrow];
-            double b0 = Bs[k4 + lk][wn0 + lrow];
-            double b1 = Bs[k4 + lk][wn0 + 16 + lrow];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            const T a0 = As[k4 + lk][wm0 + lrow];
+            const T a1 = As[k4 + lk][wm0 + 16 + lrow];
+            const T b0 = Bs[k4 + lk][wn0 + lrow];
+            const T b1 = Bs[k4 + lk][wn0 + 16 + lrow];
+            acc[0][0] = Mfma<T>::step(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::step(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::step(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::step(a1, b1, acc[1][1]);
         }
     }
 
@@ -188,41 +209,41 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
             const int col = n0 + wn0 + 16 * j + lrow;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                // v_mfma_f64_16x16x4f64 result: lane l, register r holds D[(l >> 4) + 4 r][l & 15]
-                const int row = t.m0 + wm0 + 16 * i + lk + 4 * r;
+                // lane l, register r holds D[row(l >> 4, r)][l & 15]
+                const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
                 if (row >= M) continue;
-                const double v = acc[i][j][r];
+                const T v = acc[i][j][r];
                 if (MODE == Sweep::Forward) {
                     if (row < s) {
                         Out[(int64_t)(start + row) * ldk + col] = v;
                     } else {
                         const int64_t ur = uoff + (row - s);
                         const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
-                        const double xa = U[(ta >= 0 ? (int64_t)ta : ur) * ldk + col];
-                        const double xb = U[(tb >= 0 ? (int64_t)tb : ur) * ldk + col];
-                        U[ur * ldk + col] = v + (ta >= 0 ? xa : 0.0) + (tb >= 0 ? xb : 0.0);
+                        const T xa = U[(ta >= 0 ? (int64_t)ta : ur) * ldk + col];
+                        const T xb = U[(tb >= 0 ? (int64_t)tb : ur) * ldk + col];
+                        U[ur * ldk + col] = v + (ta >= 0 ? xa : T(0)) + (tb >= 0 ? xb : T(0));
                     }
                 } else {
-                    Out[(int64_t)(start + row) * ldk + col] = v;
+                    const int64_t o = (int64_t)(start + row) * ldk + col;
+                    Out[o] = v;
+                    if (acc_mode == 1) acc64[o] = (double)v;
+                    if (acc_mode == 2) acc64[o] += (double)v;
                 }
             }
         }
     }
 }
 
-}  // namespace
-
-void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
-                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
-                     const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds, bool leaves) {
+template <typename T>
+void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+              const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
+              const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves) {
     if (ntiles <= 0) return;
     const dim3 grid(ntiles, ldk / kTileN);
-#define FCW_FWD(LDS, KIDS)                                                                                      \
-    sweep_kernel<Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b, \
-                                                                       nullptr, nullptr, node_poff, node_uoff,  \
-                                                                       contrib_own, contrib_tail, panel, X,     \
-                                                                       nullptr, Y, U)
+#define FCW_FWD(LDS, KIDS)                                                                                   \
+    sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
+        tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
+        panel, X, nullptr, Y, U, nullptr, 0)
     if (leaves)
         FCW_FWD(true, false);
     else if (idx_lds)
@@ -232,20 +253,53 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 #undef FCW_FWD
 }
 
-void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
-                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds) {
+template <typename T>
+void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+              const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const T* panel,
+              const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode) {
     if (ntiles <= 0) return;
     const dim3 grid(ntiles, ldk / kTileN);
     if (idx_lds)
-        sweep_kernel<Sweep::Backward, true, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
-                                                                             node_boff, bidx, node_poff, nullptr,
-                                                                             nullptr, nullptr, panel, Y, Z, Z, nullptr);
+        sweep_kernel<T, Sweep::Backward, true, true><<<grid, kThreads, 0, st>>>(
+            tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
+            Z, nullptr, acc64, acc_mode);
     else
-        sweep_kernel<Sweep::Backward, false, true><<<grid, kThreads, 0, st>>>(tiles, ldk, node_start, node_s, node_b,
-                                                                              node_boff, bidx, node_poff, nullptr,
-                                                                              nullptr, nullptr, panel, Y, Z, Z,
-                                                                              nullptr);
+        sweep_kernel<T, Sweep::Backward, false, true><<<grid, kThreads, 0, st>>>(
+            tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
+            Z, nullptr, acc64, acc_mode);
+}
+
+}  // namespace
+
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
+                     const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
+                     double* U, bool idx_lds, bool leaves) {
+    fwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
+                     contrib_tail, panel, X, Y, U, idx_lds, leaves);
+}
+
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
+                     const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,
+                     float* U, bool idx_lds, bool leaves) {
+    fwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
+                    contrib_tail, panel, X, Y, U, idx_lds, leaves);
+}
+
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds) {
+    bwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
+                     idx_lds, nullptr, 0);
+}
+
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+                     const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
+                     const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
+                     double* acc64, int acc_mode) {
+    bwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
+                    idx_lds, acc64, acc_mode);
 }
 
 }  // namespace fcw

@@ -23,7 +23,9 @@ typedef struct fcw_ctx fcw_ctx;
 /* Arithmetic of the two triangular sweeps per system. */
 enum {
     FCW_FP64 = 0,    /* plain FP64 panels and RHS blocks */
-    FCW_FP32_IR = 1  /* FP32 panel products, FP64 residual refinement */
+    FCW_FP32_IR = 1  /* Poisson system: FP32 panel products with FP32 accumulation, refined against the FP64
+                        residual; the heat system stays FP64 (its solution spans 1 to ~1e-141 at the
+                        164k-vertex mesh, below FP32's range) */
 };
 
 /* Scalar outputs per source, in this order (fcw_metrics_out.scalars is k x FCW_NSCALARS). */
@@ -49,7 +51,7 @@ typedef struct {
     int leaf_size;         /* nested-dissection leaf size; default 64 */
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
-    int ir_steps;          /* refinement sweeps per system in FCW_FP32_IR mode; default 2 */
+    int ir_steps;          /* FCW_FP32_IR: residual + correction solves after the first FP32 solve; default 1 */
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 4 */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
     int reserved[4];

@@ -110,6 +110,27 @@ def test_streams_and_sparse_rhs_give_identical_distances(c1):
         assert np.max(np.abs(D - ref)) <= 1e-12 * np.max(ref), (streams, sparse)
 
 
+def test_fp32_refined_poisson_meets_the_target(c2):
+    """FCW_FP32_IR: Poisson system in FP32 sweeps refined against the FP64 residual; distances
+    within 1e-5 relative of the FP64 oracle (d > 5h, as compare_fastcw_lean.py:235-239), and
+    closer with more refinement steps."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    s, mesh, ctx, H = c2
+    src = np.arange(3, mesh.n, 41)[:128]
+    Dref = H.compute_distance_batch(src)
+    far = Dref > 5 * mesh.health_h
+    errs = []
+    for steps in (0, 1, 2):
+        with HeatContext(mesh.V, mesh.F, block=128, precision="fp32_ir", ir_steps=steps) as c:
+            D = c.distances(src)
+        assert np.all(D[np.arange(src.size), src] == 0.0)
+        errs.append(np.max(rel_err(D[far], Dref[far])))
+        print("ir_steps", steps, f"rel(d>5h) {errs[-1]:.2e}", flush=True)
+    assert errs[1] < 1e-5 and errs[2] < 1e-5
+    assert errs[2] <= errs[0]
+
+
 def test_geometry_fields_euclid_and_non_lipschitz(c2):
     from oracle import geometry as og
 
