@@ -147,8 +147,18 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4, ch
                                 perim=chk["perimeters"], msd_u=chk["scalars"][:, 4], msd_w=chk["scalars"][:, 5])
     out = dict(precision=precision, sources=int(src.size), steps=int(steps), seconds=wall,
                sources_per_s=src.size / wall, ms_per_step=1e3 * wall / steps, setup_s=setup,
-               hemisphere_latency_s=setup + wall * (n_src / src.size), flagged=flags, band_overflow=ovf,
+               hemisphere_latency_s=setup + wall * (n_src / src.size),
+               end_to_end_sources_per_s=n_src / (setup + wall * (n_src / src.size)), flagged=flags, band_overflow=ovf,
                info=ctx.info, outputs=outputs)
+    # distance fields handed to the caller's host memory (fcw_distance_block: solve, shift, one
+    # device-to-host copy of k x n doubles per block), the adapter's path; PCIe-inclusive, reported
+    # beside the resident-input rate, never as it
+    nblk = 4
+    ctx.distance_block(order[:k])
+    t1 = time.perf_counter()
+    for b in range(nblk):
+        ctx.distance_block(np.resize(order, (b + 2) * k)[(b + 1) * k:])
+    out["distances_to_host_sources_per_s"] = nblk * k / (time.perf_counter() - t1)
     # per-kernel timing with HIP events on the context's stream
     ctx.set_profiling(True)
     ctx.wiring(np.resize(order, profile_blocks * k))
@@ -359,7 +369,9 @@ def main():
                      "avg_launch_ms": roof["avg_launch_ms"]},
         "cpu_baseline": None if cpu is None else {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
         "forms": {k: {"sources_per_s": v["sources_per_s"], "ms_per_step": v["ms_per_step"], "setup_s": v["setup_s"],
-                      "hemisphere_latency_s": v["hemisphere_latency_s"], "flagged": v["flagged"]}
+                      "hemisphere_latency_s": v["hemisphere_latency_s"],
+                      "end_to_end_sources_per_s": v["end_to_end_sources_per_s"],
+                      "distances_to_host_sources_per_s": v["distances_to_host_sources_per_s"], "flagged": v["flagged"]}
                   for k, v in results.items()},
         "hemisphere_latency_s": best["hemisphere_latency_s"],
         "accuracy": accuracy_report,

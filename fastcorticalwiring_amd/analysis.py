@@ -69,8 +69,8 @@ class WiringAnalysis:
         self.ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t,
                                poisson_shift=float(kw.pop("poisson_shift", 1e-10)), block=int(kw.pop("block", 256)),
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
-                               ir_steps=int(kw.pop("ir_steps", 2)), device=int(kw.pop("device", 0)),
-                               host_threads=int(kw.pop("host_threads", 0)))
+                               ir_steps=int(kw.pop("ir_steps", 1)), device=int(kw.pop("device", 0)),
+                               host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 4)))
         if kw:
             raise ValueError(f"Unsupported engine kwargs: {', '.join(sorted(kw))}")
         self.field_flag = np.zeros(self.n_vertices_full, dtype=np.uint8)

@@ -119,17 +119,18 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // Fused gradient -> normalise -> divergence (fastcw_lean.py:416-442), gathered per vertex
 // ---------------------------------------------------------------------------
 //
-// A workgroup owns kDivRows consecutive vertex rows for one 64-column slice of the RHS block.
-// It first stages, cooperatively and in one pass, every incidence (face, corner) of its rows:
-// the face's vertex rows, the gradients of its basis functions and its area, in LDS. Each wave
-// then takes whole rows with one lane per column: a row's field values are loaded kDivC
-// incidences at a time with all loads in flight together, and every face quantity comes from
-// LDS (a broadcast read), so no global load waits on another. Each row sums its incidences in
-// CSR order, so the result does not depend on scheduling.
+// A workgroup owns kDivRows consecutive vertex rows and every column of the RHS block. It first
+// stages, cooperatively and in one pass, every incidence (face, corner) of its rows: the face's
+// vertex rows, the gradients of its basis functions and its area, in LDS, shared by all column
+// slices. The work is then (row, 64-column slice) pairs, dealt to the waves round robin (with
+// 256 columns, wave w takes slice w of every row), one lane per column: a row's field values
+// are loaded kDivC incidences at a time with all loads in flight together, and every face
+// quantity comes from LDS (a broadcast read), so no global load waits on another. Each row sums
+// its incidences in CSR order, so the result does not depend on scheduling.
 //
-// The 1-D grid is laid out so that the column slices of one row group are dispatched to the
-// same XCD (workgroups are handed to the 8 XCDs round robin), where they share the staged
-// face data in that XCD's L2.
+// Workgroups are handed to the 8 XCDs round robin; the grid maps them so that each XCD walks
+// its own contiguous run of row groups, and the neighbour rows that consecutive groups share
+// are read from that XCD's L2.
 // 1/sqrt(q) for q > 0: the hardware estimate refined by two Newton steps (relative error a few
 // units in the last place), with q scaled out of the range where the estimate loses accuracy.
 // It replaces a correctly rounded square root followed by a division.
@@ -142,14 +143,14 @@ __device__ inline double rsqrt_refined(double q) {
     return tiny ? y * 0x1p+128 : y;
 }
 
-constexpr int kDivRows = 16;   // rows per workgroup (4 per wave)
+constexpr int kDivRows = 16;   // rows per workgroup
 constexpr int kDivInc = 256;   // incidences staged per pass (a pass per 256 when rows are long)
 constexpr int kDivC = 6;       // incidences in flight per row step (interior vertices have about 6)
 constexpr int kXcds = 8;
 
-__global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, const int* __restrict__ vf_ptr,
-                                                         const int* __restrict__ vf, const int* __restrict__ faces,
-                                                         const double* __restrict__ area,
+__global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, int run,
+                                                         const int* __restrict__ vf_ptr, const int* __restrict__ vf,
+                                                         const int* __restrict__ faces, const double* __restrict__ area,
                                                          const double* __restrict__ gb, const double* __restrict__ u,
                                                          double* __restrict__ out) {
     __shared__ int4 rec[kDivInc];         // vertex rows i0, i1, i2 of the face, corner a
@@ -157,19 +158,14 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
     __shared__ int rptr[kDivRows + 1];
 
     const int ncs = ldk / 64;
-    const int wg = blockIdx.x, xcd = wg % kXcds, k = wg / kXcds;
-    const int grp = (k / ncs) * kXcds + xcd, cs = k % ncs;
+    const int grp = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
     if (grp >= ngroups) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int col = cs * 64 + lane;
     const int r0 = grp * kDivRows, r1 = min(n, r0 + kDivRows);
     if (tid <= r1 - r0) rptr[tid] = vf_ptr[r0 + tid];
     __syncthreads();
     const int eb = rptr[0], ee = rptr[r1 - r0];
-
-    double acc[kDivRows / 4];
-#pragma unroll
-    for (int i = 0; i < kDivRows / 4; ++i) acc[i] = 0.0;
+    const int npairs = (r1 - r0) * ncs;
 
     for (int base = eb; base < ee; base += kDivInc) {
         const int cnt = min(kDivInc, ee - base);
@@ -190,11 +186,13 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
             for (int c = 0; c < 10; ++c) geo[tid][c] = gv[c];
         }
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < kDivRows / 4; ++i) {
-            const int r = w + 4 * i;
-            if (r0 + r >= r1) break;
+        for (int pr = w; pr < npairs; pr += 4) {
+            const int r = pr / ncs, col = (pr - r * ncs) * 64 + lane;
             const int rs = max(rptr[r], base), re = min(rptr[r + 1], base + cnt);
+            if (rs >= re) continue;  // uniform over the wave
+            // a row split over staging passes continues the running sum the lane stored before
+            double* o = out + (int64_t)(r0 + r) * ldk + col;
+            double acc = rs == rptr[r] ? 0.0 : *o;
             for (int e = rs; e < re; e += kDivC) {
                 double u0[kDivC], u1[kDivC], u2[kDivC];
 #pragma unroll
@@ -217,15 +215,16 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
                     const double qq = gx * gx + gy * gy + gz * gz;
                     const double term = -G[9] * (gx * G[0] + gy * G[1] + gz * G[2]) * rsqrt_refined(qq);
                     // |grad u| > 1e-300 (fastcw_lean.py:433) holds for every positive qq
-                    acc[i] += (e + q < re && qq > 0.0) ? term : 0.0;
+                    acc += (e + q < re && qq > 0.0) ? term : 0.0;
                 }
             }
+            *o = acc;
         }
     }
-#pragma unroll
-    for (int i = 0; i < kDivRows / 4; ++i) {
-        const int r = r0 + w + 4 * i;
-        if (r < r1) out[(int64_t)r * ldk + col] = acc[i];
+    // rows without incidences (isolated vertices) still get their zero
+    for (int pr = w; pr < npairs; pr += 4) {
+        const int r = pr / ncs, col = (pr - r * ncs) * 64 + lane;
+        if (rptr[r] == rptr[r + 1]) out[(int64_t)(r0 + r) * ldk + col] = 0.0;
     }
 }
 
@@ -340,9 +339,9 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {
     const int ngroups = (mesh.n + kDivRows - 1) / kDivRows;
-    const int padded = (ngroups + kXcds - 1) / kXcds * kXcds;  // whole rounds of the XCD mapping
-    divergence_kernel<<<dim3(padded * (ldk / 64)), 256, 0, st>>>(mesh.n, ldk, ngroups, mesh.vf_ptr, mesh.vf,
-                                                                 mesh.faces, mesh.area, mesh.gb, u, out);
+    const int run = (ngroups + kXcds - 1) / kXcds;  // row groups per XCD
+    divergence_kernel<<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf, mesh.faces,
+                                                         mesh.area, mesh.gb, u, out);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,

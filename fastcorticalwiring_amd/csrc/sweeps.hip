@@ -60,18 +60,29 @@ enum class Sweep { Forward, Backward };
 // accumulation for the refined solve). KIDS = false: every node of the launch is a leaf (no
 // child update rows to gather). Backward, acc_mode 1 / 2 also stores / adds each solution value
 // into the FP64 array acc64 (the iterate of the refined solve).
+//
+// Grid: one workgroup per (row tile, 64-column slice), laid out in 1-D so that the column slices
+// of a row tile are dispatched one after another to the same XCD (workgroups go to the 8 XCDs
+// round robin): they read the same panel rows, which the first of them brings into that XCD's
+// L2 for the others. ntiles rounds up to a multiple of 8; workgroups past the end exit.
+constexpr int kXcds = 8;
+
 template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
-    const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
+    int ntiles, const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
     const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
     T* U, double* acc64, int acc_mode) {
-    const LevelTile t = tiles[blockIdx.x];
+    const int ncol = ldk / kTileN;
+    const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int tile = (j / ncol) * kXcds + xcd;
+    if (tile >= ntiles) return;
+    const LevelTile t = tiles[tile];
     const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
     const T* __restrict__ P = panel + node_poff[t.node];
     const int* __restrict__ B = (MODE == Sweep::Backward) ? bidx + node_boff[t.node] : nullptr;
-    const int n0 = blockIdx.y * kTileN;
+    const int n0 = (j - (j / ncol) * ncol) * kTileN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
     const int M = (MODE == Sweep::Forward) ? f : s;   // output rows of this node
@@ -239,10 +250,10 @@ void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
               const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
               const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves) {
     if (ntiles <= 0) return;
-    const dim3 grid(ntiles, ldk / kTileN);
+    const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
 #define FCW_FWD(LDS, KIDS)                                                                                   \
     sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
-        tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
+        ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
         panel, X, nullptr, Y, U, nullptr, 0)
     if (leaves)
         FCW_FWD(true, false);
@@ -258,14 +269,14 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
               const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const T* panel,
               const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode) {
     if (ntiles <= 0) return;
-    const dim3 grid(ntiles, ldk / kTileN);
+    const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
     if (idx_lds)
         sweep_kernel<T, Sweep::Backward, true, true><<<grid, kThreads, 0, st>>>(
-            tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
+            ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
             Z, nullptr, acc64, acc_mode);
     else
         sweep_kernel<T, Sweep::Backward, false, true><<<grid, kThreads, 0, st>>>(
-            tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
+            ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
             Z, nullptr, acc64, acc_mode);
 }
 

@@ -51,14 +51,15 @@ def _backend_mixin(base):
           t_coef (float, default 1.0): t = t_coef * mean_edge**2 (fastcw_lean.py:485-489), or
           diffusion_length_mm (float): t = L**2 (the potpourri convention, distance_engines.py:119-124);
           poisson_shift (1e-10), block (256 sources per solve), precision ('fp64' | 'fp32_ir'),
-          leaf_size (64), device (0).
+          ir_steps (1, refinement steps of 'fp32_ir'), leaf_size (64), device (0), host_threads (0),
+          streams (1: this adapter solves one block per miss, so more workspaces would sit idle).
         Sources are solved in RHS blocks: a miss solves the next ``block`` sources of the planned
         order (default: the reference's BFS order, core_analysis.py:1384-1417), so the caller's
         one-source-at-a-time loop reads from the cache.
         """
 
         _ALLOWED = {"t_coef", "diffusion_length_mm", "poisson_shift", "block", "precision", "leaf_size", "device",
-                    "ir_steps", "host_threads"}
+                    "ir_steps", "host_threads", "streams"}
 
         def __init__(self, vertices, faces, **engine_kwargs):
             super().__init__(vertices, faces, **engine_kwargs)
@@ -103,8 +104,9 @@ def _backend_mixin(base):
             self.backend_info = info
             self._ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t, poisson_shift=shift,
                                     block=int(kw.pop("block", 256)), leaf_size=int(kw.pop("leaf_size", 64)),
-                                    precision=precision, ir_steps=int(kw.pop("ir_steps", 2)),
-                                    device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)))
+                                    precision=precision, ir_steps=int(kw.pop("ir_steps", 1)),
+                                    device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)),
+                                    streams=int(kw.pop("streams", 1)))
             self.t = self._ctx.t
             self._plan = None
             self._plan_pos = None
