@@ -114,7 +114,7 @@ def cpu_worker(workload, n_sources, out_path, seed=0):
 # --------------------------------------------------------------------------------------------
 # GPU measurement
 # --------------------------------------------------------------------------------------------
-def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=4, check_src=None):
+def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=8, check_src=None):
     from fastcorticalwiring_amd.engine import HeatContext
 
     t0 = time.perf_counter()
@@ -244,7 +244,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=3)
     ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--streams", type=int, default=4, help="RHS blocks in flight (one HIP stream each)")
+    ap.add_argument("--streams", type=int, default=8, help="RHS blocks in flight (one HIP stream each)")
     ap.add_argument("--precision", default="auto", help="fp64 | fp32_ir | auto (every form available)")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-sources", type=int, default=24)

@@ -15,6 +15,18 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfcw.so")
 
+# HIP serves a process's streams from GPU_MAX_HW_QUEUES hardware queues (4 unless set). A context
+# keeps one RHS block in flight per stream (8 by default), and blocks on more streams than queues
+# share a queue and run one after another. The HIP runtime reads the variable once, when it
+# initialises, so it is raised here, before the library is loaded; a larger value set by the
+# caller is kept, and a process that initialised HIP earlier keeps what it has.
+HW_QUEUES = 8
+try:
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "0")) < HW_QUEUES:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+except ValueError:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 FCW_FP64 = 0
 FCW_FP32_IR = 1
 

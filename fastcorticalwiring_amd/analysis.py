@@ -70,7 +70,7 @@ class WiringAnalysis:
                                poisson_shift=float(kw.pop("poisson_shift", 1e-10)), block=int(kw.pop("block", 256)),
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
                                ir_steps=int(kw.pop("ir_steps", 1)), device=int(kw.pop("device", 0)),
-                               host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 4)))
+                               host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)))
         if kw:
             raise ValueError(f"Unsupported engine kwargs: {', '.join(sorted(kw))}")
         self.field_flag = np.zeros(self.n_vertices_full, dtype=np.uint8)

@@ -528,7 +528,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->host_threads = 0;
     opt->device = 0;
     opt->ir_steps = 1;
-    opt->streams = 4;
+    opt->streams = 8;
     opt->sparse_rhs = 1;
 }
 
@@ -558,7 +558,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         if (opt.ir_steps < 0 || opt.ir_steps > 8) throw Fail("ir_steps must be in 0..8");
         if (opt.block <= 0) opt.block = 256;
         if (opt.streams <= 0) opt.streams = 1;
-        if (opt.streams > 8) throw Fail("streams must be in 1..8");
+        if (opt.streams > 16) throw Fail("streams must be in 1..16");
         if (!(opt.poisson_shift > 0.0) || !std::isfinite(opt.poisson_shift))
             throw Fail("poisson_shift must be positive and finite");
         check_faces(n, Vin, m, F);

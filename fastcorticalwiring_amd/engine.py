@@ -23,7 +23,7 @@ class HeatContext:
     """
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
-                 precision="fp64", ir_steps=1, device=0, host_threads=0, streams=4, sparse_rhs=True):
+                 precision="fp64", ir_steps=1, device=0, host_threads=0, streams=8, sparse_rhs=True):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
         if V.ndim != 2 or V.shape[1] != 3:

@@ -76,7 +76,8 @@ class _Prepared:
         self.V, self.F, self.s2o, self.o2s = mp.build_cortex_submesh(V, F, M)
         fa, fl, va = mp.face_geometry(self.V, self.F)
         self.order = mp.bfs_order_all(self.V.shape[0], self.F)
-        self.ctx = HeatContext(self.V, self.F, block=block, precision=precision, host_threads=host_threads)
+        self.ctx = HeatContext(self.V, self.F, block=block, precision=precision, host_threads=host_threads,
+                               streams=streams)
         self.ctx_streams = streams
         targets = np.array(sorted(req.scales)) * float(va.sum())
         q = int(req.n_samples_between_scales)
@@ -92,10 +93,12 @@ class ServingEngine:
     hbm_budget_bytes : admission budget for the sum of active contexts' estimated device memory
     max_active       : concurrent hemispheres on the device
     chunk_blocks     : RHS blocks a worker submits per call (smaller = finer interleaving)
+    streams          : RHS blocks in flight per hemisphere context (the active contexts share the
+                       device's hardware queues)
     """
 
     def __init__(self, hbm_budget_bytes=96 << 30, max_active=3, block=256, chunk_blocks=8, prep_threads=2,
-                 host_threads=8, precision="fp64", keep_outputs=True, prepare=None):
+                 host_threads=8, precision="fp64", keep_outputs=True, prepare=None, streams=4):
         self.hbm_budget = int(hbm_budget_bytes)
         self.max_active = int(max_active)
         self.block = int(block)
@@ -104,7 +107,9 @@ class ServingEngine:
         self.host_threads = int(host_threads)
         self.precision = precision
         self.keep_outputs = keep_outputs
-        self._prepare = prepare or (lambda req: _Prepared(req, self.block, 2, self.host_threads, self.precision))
+        self.streams = int(streams)
+        self._prepare = prepare or (lambda req: _Prepared(req, self.block, self.streams, self.host_threads,
+                                                           self.precision))
         self._lock = threading.Condition()
         self._in_use = 0
         self._active = 0
@@ -125,7 +130,7 @@ class ServingEngine:
 
     def _serve_one(self, req: HemisphereRequest, res: RequestResult, t0: float, prep_sem: threading.Semaphore):
         n_est = int(np.count_nonzero(req.cortex_mask))
-        need = estimate_device_bytes(n_est, 2 * n_est, self.block)
+        need = estimate_device_bytes(n_est, 2 * n_est, self.block, self.streams)
         res.device_bytes = need
         self._admit(need)
         res.admitted_s = time.perf_counter() - t0

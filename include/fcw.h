@@ -52,7 +52,8 @@ typedef struct {
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
     int ir_steps;          /* FCW_FP32_IR: residual + correction solves after the first FP32 solve; default 1 */
-    int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces); default 4 */
+    int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces), 1..16;
+                              default 8 (run concurrently when GPU_MAX_HW_QUEUES >= streams) */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
     int reserved[4];
 } fcw_options;
