@@ -238,17 +238,24 @@ def test_field_health_screen(c2_mesh):
     assert bad and math.isnan(a) and math.isnan(b)
 
 
+def _golden():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")) as f:
+        return json.load(f)
+
+
 def test_boundary_area_loss_fraction_known_answers():
-    """Known answers of the reference's test_vertex_subsetting.py:52-58."""
+    """Known answers of the reference's test_vertex_subsetting.py:52-58 (tests/golden fixture)."""
     from fastcorticalwiring_amd import mesh_prep
     from oracle import geometry as og
 
+    g = _golden()["boundary_area_loss_fraction"]
     for fn in (og.boundary_area_loss_fraction, mesh_prep.boundary_area_loss_fraction):
-        assert fn(10.0, 10.0) == pytest.approx(0.0, abs=1e-12)
-        assert fn(10.0, 5.0) == pytest.approx(0.19550110947788538, abs=1e-12)
-        assert fn(10.0, 0.0) == pytest.approx(0.5, abs=1e-12)
-        assert fn(10.0, 20.0) == pytest.approx(0.0, abs=1e-12)
-        assert fn(10.0, np.inf) == pytest.approx(0.0, abs=1e-12)
+        for c in g["cases"]:
+            db = float(c["dist_to_boundary"])
+            assert fn(c["radius"], db) == pytest.approx(c["expected"], abs=g["tolerance_abs"])
         assert math.isnan(fn(np.nan, 1.0)) and math.isnan(fn(0.0, 1.0))
 
 
@@ -265,8 +272,7 @@ def test_clean_vertex_samples():
 def test_twelve_scales_match_reference_list():
     from fastcorticalwiring_amd import meshes
 
-    ref = [0.002, 0.00267988, 0.00359088, 0.00481157, 0.00644721, 0.00863888, 0.01157558, 0.01551059,
-           0.02078326, 0.02784833, 0.03731509, 0.05]  # test_fastcw_lean.py:51-52
+    ref = _golden()["twelve_scales"]["values"]  # run_pipeline.py:248-267, test_fastcw_lean.py:51-52
     assert list(meshes.TWELVE_SCALES) == ref
     logspaced = 0.002 * 25.0 ** (np.arange(12) / 11.0)
     assert np.max(np.abs(np.array(ref) - logspaced)) < 5e-9  # 8-decimal rounding of the log-spaced set
@@ -305,3 +311,11 @@ def test_bfs_order_small_known_answer():
     assert mesh_prep.bfs_order_all(8, F).tolist() == expect
     F2 = np.array([[4, 0, 3], [0, 1, 3], [1, 2, 3]], np.int32)
     assert oa.bfs_order_all(5, F2) == [0, 1, 3, 4, 2]
+
+
+def test_normalize_scales_known_answer():
+    """test_vertex_subsetting.py:60-62: scales are sorted ascending (tests/golden fixture)."""
+    from fastcorticalwiring_amd import mesh_prep
+
+    g = _golden()["normalize_scales"]
+    assert tuple(mesh_prep.normalize_scales(g["input"])) == tuple(g["expected"])
