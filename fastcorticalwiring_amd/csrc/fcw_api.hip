@@ -567,6 +567,9 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->n = n;
         C->m = m;
         C->ldk = ((opt.block + fcw::kColPad - 1) / fcw::kColPad) * fcw::kColPad;
+        // the divergence addresses one RHS block with 32-bit byte offsets
+        if ((uint64_t)n * (uint64_t)C->ldk * 8u >= (1ull << 32))
+            throw Fail("an RHS block of this mesh would exceed 4 GiB; use a smaller block");
         HIPCHK(hipSetDevice(opt.device));
         double t0 = now_s();
         std::string err;

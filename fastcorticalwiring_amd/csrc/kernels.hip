@@ -120,13 +120,18 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // ---------------------------------------------------------------------------
 //
 // A workgroup owns kDivRows consecutive vertex rows and every column of the RHS block. It first
-// stages, cooperatively and in one pass, every incidence (face, corner) of its rows: the face's
-// vertex rows, the gradients of its basis functions and its area, in LDS, shared by all column
-// slices. The work is then (row, 64-column slice) pairs, dealt to the waves round robin (with
-// 256 columns, wave w takes slice w of every row), one lane per column: a row's field values
-// are loaded kDivC incidences at a time with all loads in flight together, and every face
-// quantity comes from LDS (a broadcast read), so no global load waits on another. Each row sums
-// its incidences in CSR order, so the result does not depend on scheduling.
+// stages, cooperatively and in one pass, every incidence (face f, corner a) of its rows in LDS:
+// the byte offsets of the face's three vertex rows and five coefficients from the face's
+// gradient basis (g1 = grad B1, g2 = grad B2, ga = grad Ba) and area A:
+//     |grad u|^2        = du1^2 |g1|^2 + du1 du2 (2 g1.g2) + du2^2 |g2|^2
+//     A (grad u . ga)   = du1 (A g1.ga) + du2 (A g2.ga)
+// with du1 = u1 - u0, du2 = u2 - u0, so the per-column work is two differences, two short dot
+// products and one reciprocal square root. The work is then (row, 64-column slice) pairs dealt
+// to the waves round robin (with 256 columns, wave w takes slice w of every row), one lane per
+// column: a row's field values are loaded kDivC incidences at a time with all loads in flight
+// together (32-bit offsets through a buffer resource, no address arithmetic), and every face
+// quantity comes from LDS (a broadcast read). Each row sums its incidences in CSR order, so the
+// result does not depend on scheduling.
 //
 // Workgroups are handed to the 8 XCDs round robin; the grid maps them so that each XCD walks
 // its own contiguous run of row groups, and the neighbour rows that consecutive groups share
@@ -147,14 +152,18 @@ constexpr int kDivRows = 16;   // rows per workgroup
 constexpr int kDivInc = 256;   // incidences staged per pass (a pass per 256 when rows are long)
 constexpr int kDivC = 6;       // incidences in flight per row step (interior vertices have about 6)
 constexpr int kXcds = 8;
+constexpr unsigned kBufWord3 = 0x00020000u;  // gfx9 buffer descriptor word 3 (raw 32-bit loads)
 
+// BUF: the field block is below 2 GiB, so loads go through a buffer resource with 32-bit
+// offsets; otherwise through 64-bit pointers (offsets still below 4 GiB).
+template <bool BUF>
 __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, int run,
                                                          const int* __restrict__ vf_ptr, const int* __restrict__ vf,
                                                          const int* __restrict__ faces, const double* __restrict__ area,
                                                          const double* __restrict__ gb, const double* __restrict__ u,
                                                          double* __restrict__ out) {
-    __shared__ int4 rec[kDivInc];         // vertex rows i0, i1, i2 of the face, corner a
-    __shared__ double geo[kDivInc][10];   // grad phi_a (3), grad phi_1 (3), grad phi_2 (3), area
+    __shared__ uint4 rec[kDivInc];       // byte offsets of the face's vertex rows 0, 1, 2
+    __shared__ double geo[kDivInc][5];   // A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2
     __shared__ int rptr[kDivRows + 1];
 
     const int ncs = ldk / 64;
@@ -166,6 +175,9 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
     __syncthreads();
     const int eb = rptr[0], ee = rptr[r1 - r0];
     const int npairs = (r1 - r0) * ncs;
+    const unsigned row_bytes = (unsigned)ldk * 8u;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)u, (short)0, BUF ? (int)(row_bytes * (unsigned)n) : 0, kBufWord3);
 
     for (int base = eb; base < ee; base += kDivInc) {
         const int cnt = min(kDivInc, ee - base);
@@ -175,32 +187,40 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
             const int f = code / 3, a = code - 3 * f;
             const int i0 = faces[3 * (int64_t)f], i1 = faces[3 * (int64_t)f + 1], i2 = faces[3 * (int64_t)f + 2];
             const double* g = gb + 9 * (int64_t)f;
-            double gv[10];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) gv[c] = g[3 * a + c];
-#pragma unroll
-            for (int c = 3; c < 9; ++c) gv[c] = g[c];
-            gv[9] = area[f];
-            rec[tid] = make_int4(i0, i1, i2, a);
-#pragma unroll
-            for (int c = 0; c < 10; ++c) geo[tid][c] = gv[c];
+            const double A = area[f];
+            const double ax = g[3 * a], ay = g[3 * a + 1], az = g[3 * a + 2];
+            const double g1x = g[3], g1y = g[4], g1z = g[5], g2x = g[6], g2y = g[7], g2z = g[8];
+            rec[tid] = make_uint4((unsigned)i0 * row_bytes, (unsigned)i1 * row_bytes, (unsigned)i2 * row_bytes, 0u);
+            geo[tid][0] = A * (g1x * ax + g1y * ay + g1z * az);
+            geo[tid][1] = A * (g2x * ax + g2y * ay + g2z * az);
+            geo[tid][2] = g1x * g1x + g1y * g1y + g1z * g1z;
+            geo[tid][3] = 2.0 * (g1x * g2x + g1y * g2y + g1z * g2z);
+            geo[tid][4] = g2x * g2x + g2y * g2y + g2z * g2z;
         }
         __syncthreads();
         for (int pr = w; pr < npairs; pr += 4) {
-            const int r = pr / ncs, col = (pr - r * ncs) * 64 + lane;
+            const int r = pr / ncs, cs = pr - r * ncs;
+            const unsigned cb = (unsigned)(cs * 64 + lane) * 8u;  // this lane's column, in bytes
             const int rs = max(rptr[r], base), re = min(rptr[r + 1], base + cnt);
             if (rs >= re) continue;  // uniform over the wave
             // a row split over staging passes continues the running sum the lane stored before
-            double* o = out + (int64_t)(r0 + r) * ldk + col;
+            double* o = out + (int64_t)(r0 + r) * ldk + cs * 64 + lane;
             double acc = rs == rptr[r] ? 0.0 : *o;
             for (int e = rs; e < re; e += kDivC) {
                 double u0[kDivC], u1[kDivC], u2[kDivC];
 #pragma unroll
                 for (int q = 0; q < kDivC; ++q) {
-                    const int4 R = rec[min(e + q, re - 1) - base];
-                    u0[q] = u[(int64_t)R.x * ldk + col];
-                    u1[q] = u[(int64_t)R.y * ldk + col];
-                    u2[q] = u[(int64_t)R.z * ldk + col];
+                    const uint4 R = rec[min(e + q, re - 1) - base];
+                    if (BUF) {
+                        u0[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.x + cb, 0, 0));
+                        u1[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.y + cb, 0, 0));
+                        u2[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.z + cb, 0, 0));
+                    } else {
+                        const char* ub = reinterpret_cast<const char*>(u) + cb;
+                        u0[q] = *reinterpret_cast<const double*>(ub + R.x);
+                        u1[q] = *reinterpret_cast<const double*>(ub + R.y);
+                        u2[q] = *reinterpret_cast<const double*>(ub + R.z);
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);  // keep every load of the step ahead of the arithmetic
 #pragma unroll
@@ -209,12 +229,10 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
                     const double* G = geo[min(e + q, re - 1) - base];
                     const double du1 = u1[q] - u0[q];
                     const double du2 = u2[q] - u0[q];
-                    const double gx = du1 * G[3] + du2 * G[6];
-                    const double gy = du1 * G[4] + du2 * G[7];
-                    const double gz = du1 * G[5] + du2 * G[8];
-                    const double qq = gx * gx + gy * gy + gz * gz;
-                    const double term = -G[9] * (gx * G[0] + gy * G[1] + gz * G[2]) * rsqrt_refined(qq);
+                    const double num = fma(du2, G[1], du1 * G[0]);
+                    const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
                     // |grad u| > 1e-300 (fastcw_lean.py:433) holds for every positive qq
+                    const double term = -num * rsqrt_refined(qq);
                     acc += (e + q < re && qq > 0.0) ? term : 0.0;
                 }
             }
@@ -340,8 +358,13 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {
     const int ngroups = (mesh.n + kDivRows - 1) / kDivRows;
     const int run = (ngroups + kXcds - 1) / kXcds;  // row groups per XCD
-    divergence_kernel<<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf, mesh.faces,
-                                                         mesh.area, mesh.gb, u, out);
+    const uint64_t bytes = (uint64_t)mesh.n * (uint64_t)ldk * 8u;  // below 4 GiB (checked at creation)
+    if (bytes < (1ull << 31))
+        divergence_kernel<true><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf,
+                                                                   mesh.faces, mesh.area, mesh.gb, u, out);
+    else
+        divergence_kernel<false><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf,
+                                                                    mesh.faces, mesh.area, mesh.gb, u, out);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,

@@ -61,10 +61,11 @@ enum class Sweep { Forward, Backward };
 // child update rows to gather). Backward, acc_mode 1 / 2 also stores / adds each solution value
 // into the FP64 array acc64 (the iterate of the refined solve).
 //
-// Grid: one workgroup per (row tile, 64-column slice), laid out in 1-D so that the column slices
-// of a row tile are dispatched one after another to the same XCD (workgroups go to the 8 XCDs
-// round robin): they read the same panel rows, which the first of them brings into that XCD's
-// L2 for the others. ntiles rounds up to a multiple of 8; workgroups past the end exit.
+// Grid: one workgroup per (row tile, 64-column slice), laid out in 1-D so that each XCD
+// (workgroups go to the 8 XCDs round robin) walks its own contiguous run of tiles, the column
+// slices of a tile one after another: the slices read the same panel rows, and the row tiles of
+// one node the same right-hand-side rows, which the first of them brings into that XCD's L2 for
+// the others. Workgroups past the last tile exit.
 constexpr int kXcds = 8;
 
 template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS>
@@ -76,7 +77,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     T* U, double* acc64, int acc_mode) {
     const int ncol = ldk / kTileN;
     const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
-    const int tile = (j / ncol) * kXcds + xcd;
+    const int run = (ntiles + kXcds - 1) / kXcds;  // tiles per XCD
+    const int tile = xcd * run + j / ncol;
     if (tile >= ntiles) return;
     const LevelTile t = tiles[tile];
     const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
@@ -229,10 +231,13 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                         Out[(int64_t)(start + row) * ldk + col] = v;
                     } else {
                         const int64_t ur = uoff + (row - s);
-                        const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
-                        const T xa = U[(ta >= 0 ? (int64_t)ta : ur) * ldk + col];
-                        const T xb = U[(tb >= 0 ? (int64_t)tb : ur) * ldk + col];
-                        U[ur * ldk + col] = v + (ta >= 0 ? xa : T(0)) + (tb >= 0 ? xb : T(0));
+                        T x = v;
+                        if (KIDS) {  // a leaf has no children, so no update rows pass through it
+                            const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
+                            if (ta >= 0) x += U[(int64_t)ta * ldk + col];
+                            if (tb >= 0) x += U[(int64_t)tb * ldk + col];
+                        }
+                        U[ur * ldk + col] = x;
                     }
                 } else {
                     const int64_t o = (int64_t)(start + row) * ldk + col;
@@ -250,7 +255,7 @@ void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
               const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
               const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves) {
     if (ntiles <= 0) return;
-    const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
+    const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));  // 8 runs of ceil(ntiles / 8)
 #define FCW_FWD(LDS, KIDS)                                                                                   \
     sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
         ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
