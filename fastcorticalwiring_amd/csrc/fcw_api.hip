@@ -228,7 +228,7 @@ struct fcw_ctx {
     // One system: forward then backward sweep, X (RHS) -> Z (solution) through Y/U. With a plan the
     // forward sweep visits only the plan's active supernodes (Y/U must then be the heat pair, zero on
     // every other row) and their rows are cleared again after the backward sweep.
-    void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U) {
+    void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, double* Dt = nullptr) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
             const fcw::LevelTile* tiles;
@@ -249,7 +249,7 @@ struct fcw_ctx {
             EventPair* e = ev_begin(w, kBwdGemm);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p,
-                                 bwd_idx_lds);
+                                 bwd_idx_lds, Dt, n);
             ev_end(w, e);
         }
         if (plan)
@@ -257,8 +257,9 @@ struct fcw_ctx {
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
     }
 
-    // FP32 sweeps of one system: w.X32 -> w.Z32, the solution also stored / added into acc64.
-    void solve32(Workspace& w, const float* panel, double* acc64, int acc_mode) {
+    // FP32 sweeps of one system: w.X32 -> w.Z32, the solution also stored / added into acc64 (and
+    // written source-major into Dt when set).
+    void solve32(Workspace& w, const float* panel, double* acc64, int acc_mode, double* Dt = nullptr) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
             EventPair* e = ev_begin(w, kFwdGemm);
@@ -271,7 +272,7 @@ struct fcw_ctx {
             EventPair* e = ev_begin(w, kBwdGemm);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
-                                 w.Z32.p, bwd_idx_lds, acc64, acc_mode);
+                                 w.Z32.p, bwd_idx_lds, acc64, acc_mode, Dt, n);
             ev_end(w, e);
         }
     }
@@ -281,15 +282,17 @@ struct fcw_ctx {
     void poisson_refined(Workspace& w) {
         fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p, nullptr,
                                w.X32.p);
-        solve32(w, panel_pois32.p, w.Z.p, 1);
+        solve32(w, panel_pois32.p, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
             fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p,
                                    w.Z.p, w.X32.p);
-            solve32(w, panel_pois32.p, w.Z.p, 2);
+            solve32(w, panel_pois32.p, w.Z.p, 2, it + 1 == opt.ir_steps ? w.Dt.p : nullptr);
         }
     }
 
-    // Distances for the k sources whose positions are in w.src_pos[0:k]; result in w.Dt (k x n).
+    // Distances for the k sources whose positions are in w.src_pos[0:k]: the Poisson solution in w.Z
+    // (n x ldk) and source-major in w.Dt (ldk x n), both unshifted; source j's distances are its
+    // column minus its value at its own position.
     void distances(Workspace& w, int k, const BlockPlan* plan) {
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         if (profiling) {
@@ -307,8 +310,7 @@ struct fcw_ctx {
         if (opt.precision == FCW_FP32_IR)
             poisson_refined(w);
         else
-            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p);
-        fcw::launch_shift_transpose(w.stream, n, ldk, k, w.Z.p, w.src_pos.p, w.Dt.p);
+            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p);
         if (profiling) {
             HIPCHK(hipEventRecord(ev[3], w.stream));
             HIPCHK(hipEventSynchronize(ev[3]));
@@ -393,6 +395,8 @@ struct fcw_ctx {
         md.ch_hi = nullptr;
         md.part = nullptr;
         md.chunk_area = chunk_area.p;
+        md.shift_src = nullptr;  // set by the wiring path (fields stored unshifted)
+        md.shift_ld = ldk;
         return md;
     }
 
@@ -755,7 +759,7 @@ int fcw_distance_block(fcw_ctx* C, const int* src, int k, double* out) {
         std::vector<BlockPlan> plans = C->make_plans(pos);
         C->distances(w, k, C->opt.sparse_rhs ? &plans[0] : nullptr);
         if (!C->Dout.p) C->Dout.alloc((size_t)C->n * C->ldk, C->dev_bytes);
-        fcw::launch_gather_rows(w.stream, C->n, k, w.Dt.p, C->iperm.p, C->Dout.p);
+        fcw::launch_gather_rows(w.stream, C->n, C->ldk, k, w.Dt.p, w.Z.p, w.src_pos.p, C->iperm.p, C->Dout.p);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(w.stream));
         HIPCHK(hipMemcpy(out, C->Dout.p, sizeof(double) * (size_t)k * C->n, hipMemcpyDeviceToHost));
@@ -847,6 +851,7 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
             md.ch_lo = w.ch_lo.p;
             md.ch_hi = w.ch_hi.p;
             md.part = w.cpart.p;
+            md.shift_src = w.Z.p;
             fcw::launch_metrics(w.stream, k, mdv, md, w.Dt.p, w.src_pos.p, C->outputs_at((size_t)b0), true);
             C->ev_end(w, e);
             HIPCHK(hipGetLastError());

@@ -76,6 +76,10 @@ struct MetricsDev {
     const float* ch_hi;        // k x nch, rounded up
     const double* part;        // k x kChunkStats x nch
     const double* chunk_area;  // nch: area of the chunk's faces
+    // Fields are stored unshifted when shift_src is set: source j's distances are
+    // Dt[j][p] - shift_src[src_pos[j] * shift_ld + j] (its solution at its own position).
+    const double* shift_src;
+    int shift_ld;
 };
 
 // Partial record of chunk_stats: sum d (d > eps), sum d*w, sum w, max finite d, min finite d on the
@@ -107,7 +111,10 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
                      double* U, bool idx_lds, bool leaves);
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds);
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds,
+                     double* Dt, int ldd);
+// Backward sweeps with Dt set also write the FP64 solution source-major, Dt[col * ldd + position]
+// (unshifted; consumers subtract the source's own value).
 // FP32 forms of the sweeps (refined solve): FP32 panels and right-hand sides, FP32 matrix-core
 // accumulation; the backward sweep also stores (acc_mode 1) or adds (2) its solution into acc64.
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -117,15 +124,15 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
-                     double* acc64, int acc_mode);
+                     double* acc64, int acc_mode, double* Dt, int ldd);
 // r = b - (L + shift M) x in FP64 (x null: r = b), rounded to FP32 (the refined solve's RHS).
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r);
 void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count);
 void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out);
-void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
-                            double* Dt);
-void launch_gather_rows(hipStream_t st, int n, int k, const double* Dt, const int* iperm, double* out);
+// out[j][v] = Dt[j][iperm[v]] - phi[src_pos[j]][j]: fields in caller vertex order, 0 at the source.
+void launch_gather_rows(hipStream_t st, int n, int ldk, int k, const double* Dt, const double* phi, const int* src_pos,
+                        const int* iperm, double* out);
 void launch_chunk_area(hipStream_t st, int m, const double* larea, double* chunk_area);
 void launch_chunk_stats(hipStream_t st, int k, int ldk, int nch, const double* Z, const int* src_pos, const double* V,
                         const double* vw, const int* isb, const int* cvptr, const int* cverts, const int* cnown,

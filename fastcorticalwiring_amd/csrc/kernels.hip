@@ -286,34 +286,14 @@ __global__ void to_float_kernel(const double* __restrict__ src, float* __restric
         dst[i] = (float)src[i];
 }
 
-// Dt[j][p] = phi[p][j] - phi[src_j][j]   (source-major distance fields)
-__global__ void shift_transpose_kernel(int n, int ldk, int k, const double* __restrict__ phi,
-                                       const int* __restrict__ src_pos, double* __restrict__ Dt) {
-    __shared__ double tile[32][33];
-    __shared__ double base[32];
-    const int p0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
-    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
-    if (ty == 0) {
-        const int j = j0 + tx;
-        base[tx] = (j < k) ? phi[(int64_t)src_pos[j] * ldk + j] : 0.0;
-    }
-    __syncthreads();
-    for (int yy = ty; yy < 32; yy += 8) {
-        const int p = p0 + yy, j = j0 + tx;
-        tile[yy][tx] = (p < n && j < k) ? phi[(int64_t)p * ldk + j] - base[tx] : 0.0;
-    }
-    __syncthreads();
-    for (int yy = ty; yy < 32; yy += 8) {
-        const int j = j0 + yy, p = p0 + tx;
-        if (j < k && p < n) Dt[(int64_t)j * n + p] = tile[tx][yy];
-    }
-}
-
-__global__ void gather_rows_kernel(int n, int k, const double* __restrict__ Dt, const int* __restrict__ iperm,
+// out[j][v] = Dt[j][iperm[v]] - phi[src_pos[j]][j] (fastcw_lean.py:526: each field minus its value at
+// the source), rows of Dt in positions, rows of out in the caller's vertex order.
+__global__ void gather_rows_kernel(int n, int ldk, int k, const double* __restrict__ Dt, const double* __restrict__ phi,
+                                   const int* __restrict__ src_pos, const int* __restrict__ iperm,
                                    double* __restrict__ out) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
-    if (v < n && j < k) out[(int64_t)j * n + v] = Dt[(int64_t)j * n + iperm[v]];
+    if (v < n && j < k) out[(int64_t)j * n + v] = Dt[(int64_t)j * n + iperm[v]] - phi[(int64_t)src_pos[j] * ldk + j];
 }
 
 
@@ -377,13 +357,9 @@ void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t coun
     to_float_kernel<<<grid1(count, 256), 256, 0, st>>>(src, dst, count);
 }
 
-void launch_shift_transpose(hipStream_t st, int n, int ldk, int k, const double* phi, const int* src_pos,
-                            double* Dt) {
-    shift_transpose_kernel<<<dim3((n + 31) / 32, (k + 31) / 32), dim3(32, 8), 0, st>>>(n, ldk, k, phi, src_pos, Dt);
-}
-
-void launch_gather_rows(hipStream_t st, int n, int k, const double* Dt, const int* iperm, double* out) {
-    gather_rows_kernel<<<dim3((n + 255) / 256, k), 256, 0, st>>>(n, k, Dt, iperm, out);
+void launch_gather_rows(hipStream_t st, int n, int ldk, int k, const double* Dt, const double* phi, const int* src_pos,
+                        const int* iperm, double* out) {
+    gather_rows_kernel<<<dim3((n + 255) / 256, k), 256, 0, st>>>(n, ldk, k, Dt, phi, src_pos, iperm, out);
 }
 
 

@@ -124,7 +124,7 @@ struct FaceBatch {
 // across lanes); otherwise base + 64 u + lane (each step u spans a 64-face patch: few bands).
 template <bool CONSEC>
 __device__ inline void load_faces(const int* __restrict__ faces, const double* __restrict__ larea,
-                                  const double* __restrict__ d, int m, int base, int lane, FaceBatch& fb) {
+                                  const double* __restrict__ d, double sh, int m, int base, int lane, FaceBatch& fb) {
     int i0[UNR], i1[UNR], i2[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
@@ -138,9 +138,9 @@ __device__ inline void load_faces(const int* __restrict__ faces, const double* _
     }
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
-        fb.d0[u] = d[i0[u]];
-        fb.d1[u] = d[i1[u]];
-        fb.d2[u] = d[i2[u]];
+        fb.d0[u] = d[i0[u]] - sh;
+        fb.d1[u] = d[i1[u]] - sh;
+        fb.d2[u] = d[i2[u]] - sh;
     }
     __builtin_amdgcn_sched_barrier(0);  // every load of the step is issued before any use
 }
@@ -197,7 +197,7 @@ struct BandRef {
 };
 
 __device__ void band_sums(const BandRef& B, double r, bool want_p, const double* __restrict__ pool, int64_t pstride,
-                          const int* __restrict__ faces, const double* __restrict__ d,
+                          const int* __restrict__ faces, const double* __restrict__ d, double sh,
                           const double* __restrict__ larea, const double* __restrict__ g11,
                           const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
                           int nbins, double& sA, double& sdA, double& sP) {
@@ -244,9 +244,9 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
                 double d0[U], d1[U], d2[U], e11[U], e12[U], e22[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    d0[u] = d[i1[u]];
-                    d1[u] = d[i2[u]];
-                    d2[u] = d[i3[u]];
+                    d0[u] = d[i1[u]] - sh;
+                    d1[u] = d[i2[u]] - sh;
+                    d2[u] = d[i3[u]] - sh;
                     e11[u] = g11[f[u]];
                     e12[u] = g12[f[u]];
                     e22[u] = g22[f[u]];
@@ -259,7 +259,7 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
         }
     } else {
         for (int f = lane; f < m; f += kWave) {
-            const double d0 = d[faces[3 * f]], d1 = d[faces[3 * f + 1]], d2 = d[faces[3 * f + 2]];
+            const double d0 = d[faces[3 * f]] - sh, d1 = d[faces[3 * f + 1]] - sh, d2 = d[faces[3 * f + 2]] - sh;
             if (!(isfinite(d0) && isfinite(d1) && isfinite(d2))) continue;
             double fa = d0, fb = d1, fc = d2;
             sort3(fa, fb, fc);
@@ -281,6 +281,9 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const int n = mesh.n, m = mesh.m;
     const int* __restrict__ faces = mesh.faces;
     const double* __restrict__ d = Dt + (int64_t)j * n;
+    // the field's value at its own source, subtracted from every value read (0 for fields stored
+    // shifted already)
+    const double sh = md.shift_src ? md.shift_src[(int64_t)src_pos[j] * md.shift_ld + j] : 0.0;
     const int S = md.S, q = md.q, NE = (S > 0 ? (S - 1) * q : 0);
     const int NB = S + NE;  // bands: scale s at s (q + 1), its samples (s, jj) at s (q + 1) + 1 + jj
     double* scal = out.scal + (int64_t)j * kSlotCount;
@@ -352,7 +355,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         for (int u = 0; u < UNR; ++u) {
             const int p = p0 + u * NT;
             const int ps = p < n ? p : 0;
-            dv[u] = d[ps];
+            dv[u] = d[ps] - sh;
             vx[u] = mesh.V[3 * ps];
             vy[u] = mesh.V[3 * ps + 1];
             vz[u] = mesh.V[3 * ps + 2];
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     block_reduce(red, v4, i4, 3, -1);
     const double dglob = v4[3];
     for (int i = md.part ? md.nb : tid; i < md.nb; i += NT) {
-        const double dv = d[md.boundary[i]];
+        const double dv = d[md.boundary[i]] - sh;
         if (isfinite(dv)) {
             b4[0] = fmin(b4[0], dv);
             bi4[0] += 1;
@@ -501,7 +504,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         if (have_ch && (double)gch_lo[base / (kWave * UNR)] >= cut) continue;  // starts beyond every band
         FaceBatch fb;
-        load_faces<true>(faces, mesh.larea, d, m, base, lane, fb);
+        load_faces<true>(faces, mesh.larea, d, sh, m, base, lane, fb);
         double cmin = INFINITY, cmax = -INFINITY;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
@@ -723,7 +726,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             if (chi < skip_lo || clo >= skip_hi) continue;  // meets no band
         }
         FaceBatch fb;
-        load_faces<false>(faces, mesh.larea, d, m, base, lane, fb);
+        load_faces<false>(faces, mesh.larea, d, sh, m, base, lane, fb);
         int ia[UNR], ib[UNR];
         double sa_[UNR], sb_[UNR], sc_[UNR];
         bool anyf = false;
@@ -809,7 +812,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         double r_lo = br_lo, r_hi = br_hi;
         auto eval = [&](double rr, bool want_p, double& A_out, double& dA_out, double& P_out) {
             double sA, sdA, sP;
-            band_sums(B, rr, want_p, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb,
+            band_sums(B, rr, want_p, pool, pstride, faces, d, sh, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb,
                       nbins, sA, sdA, sP);
             A_out = Ain + wave_sum(sA);
             dA_out = wave_sum(sdA);
@@ -878,7 +881,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             if (lane == 0) atomicOr(&s_ovf, 4);
         }
         double sA, sdA, sP;
-        band_sums(B, rr, false, pool, pstride, faces, d, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
+        band_sums(B, rr, false, pool, pstride, faces, d, sh, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
                   sdA, sP);
         const double A = Hmax[min(B.kL, nbins)] + wave_sum(sA);
         if (lane == 0) U.x.acc[e] = A;

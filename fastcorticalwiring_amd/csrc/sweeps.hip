@@ -59,7 +59,8 @@ enum class Sweep { Forward, Backward };
 // T: value type of panels, right-hand sides and solution (double, or float with FP32
 // accumulation for the refined solve). KIDS = false: every node of the launch is a leaf (no
 // child update rows to gather). Backward, acc_mode 1 / 2 also stores / adds each solution value
-// into the FP64 array acc64 (the iterate of the refined solve).
+// into the FP64 array acc64 (the iterate of the refined solve), and with Dt set the (FP64)
+// solution value is also written source-major, Dt[col * ldd + position], for the metric kernel.
 //
 // Grid: one workgroup per (row tile, 64-column slice), laid out in 1-D so that each XCD
 // (workgroups go to the 8 XCDs round robin) walks its own contiguous run of tiles, the column
@@ -68,13 +69,13 @@ enum class Sweep { Forward, Backward };
 // the others. Workgroups past the last tile exit.
 constexpr int kXcds = 8;
 
-template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS>
+template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS, bool DT = false>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     int ntiles, const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
     const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
-    T* U, double* acc64, int acc_mode) {
+    T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd) {
     const int ncol = ldk / kTileN;
     const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
     const int run = (ntiles + kXcds - 1) / kXcds;  // tiles per XCD
@@ -90,8 +91,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const int M = (MODE == Sweep::Forward) ? f : s;   // output rows of this node
     const int K = (MODE == Sweep::Forward) ? s : f;   // depth
 
-    __shared__ T As[kTileK][kTileM + kPad];
-    __shared__ T Bs[kTileK][kTileN + kPad];
+    static_assert(kTileM == kTileN, "A and B stages share one LDS array");
+    __shared__ T sbuf[2][kTileK][kTileM + kPad];  // A and B stages; the epilogue's transpose after the loop
+    T(&As)[kTileK][kTileM + kPad] = sbuf[0];
+    T(&Bs)[kTileK][kTileN + kPad] = sbuf[1];
     __shared__ int sidx[2 * kIdxCap];
     // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
     // contributor rows for each of the node's s own rows.
@@ -214,19 +217,19 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         }
     }
 
-    const int64_t uoff = (MODE == Sweep::Forward) ? node_uoff[t.node] : 0;
+    if (MODE == Sweep::Forward) {
+        const int64_t uoff = node_uoff[t.node];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 2; ++i) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int col = n0 + wn0 + 16 * j + lrow;
+            for (int j = 0; j < 2; ++j) {
+                const int col = n0 + wn0 + 16 * j + lrow;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                // lane l, register r holds D[row(l >> 4, r)][l & 15]
-                const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
-                if (row >= M) continue;
-                const T v = acc[i][j][r];
-                if (MODE == Sweep::Forward) {
+                for (int r = 0; r < 4; ++r) {
+                    // lane l, register r holds D[row(l >> 4, r)][l & 15]
+                    const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
+                    if (row >= M) continue;
+                    const T v = acc[i][j][r];
                     if (row < s) {
                         Out[(int64_t)(start + row) * ldk + col] = v;
                     } else {
@@ -239,14 +242,72 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                         }
                         U[ur * ldk + col] = x;
                     }
-                } else {
-                    const int64_t o = (int64_t)(start + row) * ldk + col;
-                    Out[o] = v;
-                    if (acc_mode == 1) acc64[o] = (double)v;
-                    if (acc_mode == 2) acc64[o] += (double)v;
                 }
             }
         }
+        return;
+    }
+
+    // Backward: store one solution value (and the refined iterate); returns it in FP64, 0 past M.
+    auto emit = [&](int i, int j, int r) -> double {
+        const int col = n0 + wn0 + 16 * j + lrow;
+        const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
+        if (row >= M) return 0.0;
+        const T v = acc[i][j][r];
+        const int64_t o = (int64_t)(start + row) * ldk + col;
+        Out[o] = v;
+        double x = (double)v;
+        if (acc_mode == 1) acc64[o] = x;
+        if (acc_mode == 2) {
+            x += acc64[o];
+            acc64[o] = x;
+        }
+        return x;
+    };
+    if (!DT) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) (void)emit(i, j, r);
+    } else if (sizeof(T) == sizeof(double)) {
+        // Source-major copy through LDS, half a tile (32 rows x 64 columns) at a time, so each
+        // column's 32 rows go out as one 256-byte run.
+        double* stg = reinterpret_cast<double*>(&sbuf[0][0][0]);
+        static_assert(2 * kTileK * (kTileM + kPad) >= 64 * 33, "transpose stage fits the A/B stages");
+#pragma unroll 1
+        for (int h = 0; h < 2; ++h) {
+            __syncthreads();  // the stages (or the previous half) are no longer read
+            if (wm0 == 32 * h) {
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            stg[(wn0 + 16 * j + lrow) * 33 + 16 * i + Mfma<T>::row(lk, r)] = emit(i, j, r);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int idx = tid + q * kThreads;
+                const int lc = idx >> 5, lr = idx & 31;
+                const int row = t.m0 + 32 * h + lr;
+                if (row < M) Dt[(int64_t)(n0 + lc) * ldd + start + row] = stg[lc * 33 + lr];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double x = emit(i, j, r);
+                    const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
+                    if (row < M) Dt[(int64_t)(n0 + wn0 + 16 * j + lrow) * ldd + start + row] = x;
+                }
     }
 }
 
@@ -259,7 +320,7 @@ void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 #define FCW_FWD(LDS, KIDS)                                                                                   \
     sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
         ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
-        panel, X, nullptr, Y, U, nullptr, 0)
+        panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0)
     if (leaves)
         FCW_FWD(true, false);
     else if (idx_lds)
@@ -272,17 +333,22 @@ void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 template <typename T>
 void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
               const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const T* panel,
-              const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode) {
+              const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode, double* Dt, int ldd) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
-    if (idx_lds)
-        sweep_kernel<T, Sweep::Backward, true, true><<<grid, kThreads, 0, st>>>(
-            ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
-            Z, nullptr, acc64, acc_mode);
+#define FCW_BWD(LDS, DTW)                                                                                       \
+    sweep_kernel<T, Sweep::Backward, LDS, true, DTW><<<grid, kThreads, 0, st>>>(                               \
+        ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, \
+        Z, nullptr, acc64, acc_mode, Dt, ldd)
+    if (idx_lds && Dt)
+        FCW_BWD(true, true);
+    else if (idx_lds)
+        FCW_BWD(true, false);
+    else if (Dt)
+        FCW_BWD(false, true);
     else
-        sweep_kernel<T, Sweep::Backward, false, true><<<grid, kThreads, 0, st>>>(
-            ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z,
-            Z, nullptr, acc64, acc_mode);
+        FCW_BWD(false, false);
+#undef FCW_BWD
 }
 
 }  // namespace
@@ -305,17 +371,18 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
-                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds) {
+                     const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds,
+                     double* Dt, int ldd) {
     bwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
-                     idx_lds, nullptr, 0);
+                     idx_lds, nullptr, 0, Dt, ldd);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
-                     double* acc64, int acc_mode) {
+                     double* acc64, int acc_mode, double* Dt, int ldd) {
     bwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
-                    idx_lds, acc64, acc_mode);
+                    idx_lds, acc64, acc_mode, Dt, ldd);
 }
 
 }  // namespace fcw
