@@ -27,7 +27,9 @@ constexpr int kMaxScales = 32;
 constexpr int kMaxExtras = 256;
 constexpr int kMaxBins = 1536;
 constexpr int kMaxChunks = 2048;  // 256-face chunks with recorded distance bounds (larger meshes: no skipping)
-constexpr int kMetricThreads = 1024;
+// 512 threads per source: a workgroup then holds half a CU's registers, so the phases' latency
+// overlaps other streams' work on the same CU (1024: 90.5-91.5 k sources/s, 512: 93.3 k at L7)
+constexpr int kMetricThreads = 512;
 
 // Per-source scalar outputs, in this order (double each).
 enum ScalarSlot {

@@ -1,4 +1,4 @@
-// Per-source wiring metrics on gfx950: one 1024-thread workgroup per distance
+// Per-source wiring metrics on gfx950: one 512-thread workgroup per distance
 // field (core_analysis.py:1558-1609, lean_geometry.py:117-357).
 //
 //   phase 0  vertex pass: health screen, MSD (unweighted / area-weighted),
