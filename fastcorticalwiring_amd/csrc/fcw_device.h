@@ -28,7 +28,8 @@ constexpr int kMaxExtras = 256;
 constexpr int kMaxBins = 1536;
 constexpr int kMaxChunks = 2048;  // 256-face chunks with recorded distance bounds (larger meshes: no skipping)
 // 512 threads per source: a workgroup then holds half a CU's registers, so the phases' latency
-XX
+// overlaps other streams' work on the same CU (148k-vertex surface, 256-source blocks: 1024 threads
+// 90.5-91.5 k sources/s, 512 threads 93.3 k)
 constexpr int kMetricThreads = 512;
 
 // Per-source scalar outputs, in this order (double each).
