@@ -205,7 +205,8 @@ struct fcw_ctx {
     void alloc_workspace(Workspace& w) {
         const size_t ld = (size_t)ldk;
         HIPCHK(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-        const size_t urows = std::max<size_t>(1, (size_t)sym.urows);
+        // update rows plus one row that stays zero: a missing child contribution reads it
+        const size_t urows = (size_t)sym.urows + 1;
         w.X.alloc((size_t)n * ld, dev_bytes);
         w.Yh.alloc((size_t)n * ld, dev_bytes);
         w.Yp.alloc((size_t)n * ld, dev_bytes);
@@ -219,9 +220,11 @@ struct fcw_ctx {
             w.Y32.alloc((size_t)n * ld, dev_bytes);
             w.Z32.alloc((size_t)n * ld, dev_bytes);
             w.U32.alloc(urows * ld, dev_bytes);
+            HIPCHK(hipMemsetAsync(w.U32.p + (size_t)sym.urows * ld, 0, ld * sizeof(float), w.stream));
         }
         fcw::launch_fill(w.stream, w.Yh.p, (int64_t)n * ldk, 0.0);
         fcw::launch_fill(w.stream, w.Uh.p, (int64_t)urows * ldk, 0.0);
+        fcw::launch_fill(w.stream, w.Up.p + (size_t)sym.urows * ld, ldk, 0.0);
         HIPCHK(hipStreamSynchronize(w.stream));
     }
 
@@ -242,7 +245,8 @@ struct fcw_ctx {
             }
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
-                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0);
+                                 node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0,
+                                 (int)sym.urows);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
@@ -265,7 +269,7 @@ struct fcw_ctx {
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, fwd_tiles.p + fwd_tile_off[l], fwd_tile_off[l + 1] - fwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, cown.p, ctail.p, panel,
-                                 w.X32.p, w.Y32.p, w.U32.p, fwd_idx_lds, l == 0);
+                                 w.X32.p, w.Y32.p, w.U32.p, fwd_idx_lds, l == 0, (int)sym.urows);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
     const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
-    T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd) {
+    T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd, int zrow) {
     const int ncol = ldk / kTileN;
     const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
     const int run = (ntiles + kXcds - 1) / kXcds;  // tiles per XCD
@@ -141,12 +141,11 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 keep_b |= (l < s) ? (1u << i) : 0u;
                 rb[i] = Bsrc[(int64_t)pos * ldk + c];
                 if (KIDS) {
+                    // a missing contributor reads the zero row
                     const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * pos];
                     const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * pos + 1];
-                    bit_u |= (ca >= 0) ? (1u << i) : 0u;
-                    bit_v |= (cb >= 0) ? (1u << i) : 0u;
-                    ru[i] = U[(int64_t)max(ca, 0) * ldk + c];  // update row 0 stands in when unused
-                    rv[i] = U[(int64_t)max(cb, 0) * ldk + c];
+                    ru[i] = U[(int64_t)(ca >= 0 ? ca : zrow) * ldk + c];
+                    rv[i] = U[(int64_t)(cb >= 0 ? cb : zrow) * ldk + c];
                 }
             } else {
                 // [Y_S ; -Z_B] rows: own rows from Y, boundary rows negated from Z
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
             const int e = tid + i * kThreads;
             T v;
             if (MODE == Sweep::Forward && KIDS)
-                v = rb[i] - (((bit_u >> i) & 1u) ? ru[i] : T(0)) - (((bit_v >> i) & 1u) ? rv[i] : T(0));
+                v = rb[i] - ru[i] - rv[i];
             else if (MODE == Sweep::Forward)
                 v = rb[i];
             else
@@ -224,9 +223,32 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 const int col = n0 + wn0 + 16 * j + lrow;
+                // lane l, register r holds D[row(l >> 4, r)][l & 15]. Own rows go to Y; boundary rows
+                // to U, plus the children's update rows passing through (tails). The loads of the
+                // four values' tails are issued together and unconditionally (rows that are not
+                // boundary rows read in-bounds stand-ins, missing tails the zero row), so only the
+                // stores depend on the row.
+                T xa[4], xb[4];
+                if (KIDS) {  // a leaf has no children, so no update rows pass through it
+                    int ta[4], tb[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
+                        const bool bnd = row >= s && row < M;
+                        const int64_t ur = bnd ? uoff + (row - s) : 0;
+                        ta[r] = ctail[2 * ur];
+                        tb[r] = ctail[2 * ur + 1];
+                        ta[r] = (bnd && ta[r] >= 0) ? ta[r] : zrow;
+                        tb[r] = (bnd && tb[r] >= 0) ? tb[r] : zrow;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        xa[r] = U[(int64_t)ta[r] * ldk + col];
+                        xb[r] = U[(int64_t)tb[r] * ldk + col];
+                    }
+                }
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    // lane l, register r holds D[row(l >> 4, r)][l & 15]
                     const int row = t.m0 + wm0 + 16 * i + Mfma<T>::row(lk, r);
                     if (row >= M) continue;
                     const T v = acc[i][j][r];
@@ -234,13 +256,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                         Out[(int64_t)(start + row) * ldk + col] = v;
                     } else {
                         const int64_t ur = uoff + (row - s);
-                        T x = v;
-                        if (KIDS) {  // a leaf has no children, so no update rows pass through it
-                            const int ta = ctail[2 * ur], tb = ctail[2 * ur + 1];
-                            if (ta >= 0) x += U[(int64_t)ta * ldk + col];
-                            if (tb >= 0) x += U[(int64_t)tb * ldk + col];
-                        }
-                        U[ur * ldk + col] = x;
+                        U[ur * ldk + col] = KIDS ? v + xa[r] + xb[r] : v;
                     }
                 }
             }
@@ -314,13 +330,14 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 template <typename T>
 void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
               const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
-              const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves) {
+              const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves,
+              int zrow) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));  // 8 runs of ceil(ntiles / 8)
 #define FCW_FWD(LDS, KIDS)                                                                                   \
     sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
         ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
-        panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0)
+        panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0, zrow)
     if (leaves)
         FCW_FWD(true, false);
     else if (idx_lds)
@@ -339,7 +356,7 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 #define FCW_BWD(LDS, DTW)                                                                                       \
     sweep_kernel<T, Sweep::Backward, LDS, true, DTW><<<grid, kThreads, 0, st>>>(                               \
         ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, \
-        Z, nullptr, acc64, acc_mode, Dt, ldd)
+        Z, nullptr, acc64, acc_mode, Dt, ldd, 0)
     if (idx_lds && Dt)
         FCW_BWD(true, true);
     else if (idx_lds)
@@ -356,17 +373,17 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds, bool leaves) {
+                     double* U, bool idx_lds, bool leaves, int zrow) {
     fwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
-                     contrib_tail, panel, X, Y, U, idx_lds, leaves);
+                     contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow);
 }
 
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,
-                     float* U, bool idx_lds, bool leaves) {
+                     float* U, bool idx_lds, bool leaves, int zrow) {
     fwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
-                    contrib_tail, panel, X, Y, U, idx_lds, leaves);
+                    contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
