@@ -100,8 +100,22 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     // contributor rows for each of the node's s own rows.
     if (IDX_LDS) {
         if (MODE == Sweep::Backward) {
-            // source row of every depth index: own rows as positions, boundary rows as -(pos + 1)
-            for (int i = tid; i < f; i += kThreads) sidx[i] = i < s ? start + i : -(B[i - s] + 1);
+            // source row of every depth index: own rows as positions, boundary rows as -(pos + 1).
+            // The boundary-map loads are issued together from clamped indices (only the uniform
+            // per-workgroup bounds are branched on), then written.
+            constexpr int R = 2 * kIdxCap / kThreads;
+            if (b > 0) {
+                int q[R];
+#pragma unroll
+                for (int u = 0; u < R; ++u) q[u] = B[min(max(tid + u * kThreads - s, 0), b - 1)];
+#pragma unroll
+                for (int u = 0; u < R; ++u) {
+                    const int i = tid + u * kThreads;
+                    if (i < f) sidx[i] = i < s ? start + i : -(q[u] + 1);
+                }
+            } else {
+                for (int i = tid; i < f; i += kThreads) sidx[i] = start + i;
+            }
         } else if (KIDS) {
             for (int i = tid; i < 2 * s; i += kThreads) sidx[i] = cown[2 * start + i];
         }
