@@ -47,7 +47,8 @@ class Options(ctypes.Structure):
         ("ir_steps", ctypes.c_int),
         ("streams", ctypes.c_int),
         ("sparse_rhs", ctypes.c_int),
-        ("reserved", ctypes.c_int * 4),
+        ("robust", ctypes.c_int),
+        ("reserved", ctypes.c_int * 3),
     ]
 
 
@@ -67,6 +68,10 @@ class Info(ctypes.Structure):
         ("mean_edge", ctypes.c_double),
         ("setup_seconds", ctypes.c_double * 4),
         ("device_bytes", ctypes.c_int64),
+        ("robust", ctypes.c_int),
+        ("intrinsic_flips", ctypes.c_int),
+        ("intrinsic_skipped", ctypes.c_int),
+        ("mollify_delta", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -92,7 +97,7 @@ EXPORTED_SYMBOLS = (
     "fcw_default_options", "fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block",
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
-    "fcw_calibration_copy",
+    "fcw_calibration_copy", "fcw_host_intrinsic",
 )
 
 _lib = None
@@ -132,9 +137,11 @@ def lib():
     L.fcw_calibration_copy.argtypes = [ctypes.c_int64, ctypes.c_int]
     L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+    L.fcw_host_intrinsic.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp, _dp, _dp]
     for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
-                 "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy"):
+                 "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy",
+                 "fcw_host_intrinsic"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -188,3 +195,19 @@ def host_factor_solve(V, F, l_scale, m_scale, B, leaf_size=64):
                                       int(leaf_size), dptr(B), B.shape[1], dptr(X), dptr(stats)))
     return X, dict(nnodes=int(stats[0]), nlevels=int(stats[1]), factor_nnz=int(stats[2]),
                    panel_elems=int(stats[3]))
+
+
+def host_intrinsic(V, F, mollify=1e-6):
+    """CPU-only: the intrinsic Delaunay triangulation the robust operators use. Returns (faces m x 3,
+    lengths m x 3 opposite each corner, area m, gram m x 6, stats)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    m = F.shape[0]
+    faces = np.empty((m, 3), np.int32)
+    length = np.empty((m, 3))
+    area = np.empty(m)
+    gram = np.empty((m, 6))
+    stats = np.zeros(3)
+    check(lib().fcw_host_intrinsic(dptr(V), V.shape[0], iptr(F), m, float(mollify), iptr(faces), dptr(length),
+                                   dptr(area), dptr(gram), dptr(stats)))
+    return faces, length, area, gram, dict(flips=int(stats[0]), skipped=int(stats[1]), delta=float(stats[2]))

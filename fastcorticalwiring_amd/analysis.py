@@ -65,12 +65,14 @@ class WiringAnalysis:
             self.t_coef = (L / self.mean_edge_length) ** 2
         else:
             self.t_coef = float(kw.pop("t_coef", 1.0))
+        self.use_robust = bool(kw.pop("use_robust", False))
         self.prep_seconds = time.perf_counter() - t0
         self.ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t,
                                poisson_shift=float(kw.pop("poisson_shift", 1e-10)), block=int(kw.pop("block", 256)),
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
                                ir_steps=int(kw.pop("ir_steps", 1)), device=int(kw.pop("device", 0)),
-                               host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)))
+                               host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)),
+                               robust=self.use_robust)
         if kw:
             raise ValueError(f"Unsupported engine kwargs: {', '.join(sorted(kw))}")
         self.field_flag = np.zeros(self.n_vertices_full, dtype=np.uint8)
@@ -188,7 +190,7 @@ class WiringAnalysis:
         return {
             "geometry_method": GEOMETRY_METHOD,
             "engine": "mi355x_heat",
-            "use_robust": False,
+            "use_robust": self.use_robust,
             "diffusion_length_mm": self.diffusion_length_mm,
             "t_coef": self.t_coef,
             "mean_edge_length_mm": self.mean_edge_length,

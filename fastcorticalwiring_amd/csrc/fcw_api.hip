@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -111,7 +112,11 @@ struct fcw_ctx {
     std::vector<int> node_of_pos;
 
     // mesh and operators (positions)
-    DBuf<double> V, area, gb, larea, g11, g12, g22, lval, mass;
+    DBuf<double> V, area, gram, larea, g11, g12, g22, lval, mass;
+    // robust mode: the operator triangulation (intrinsic Delaunay) beside the surface's faces
+    bool robust = false;
+    DBuf<int> op_faces;
+    DBuf<double> op_area, op_gram;
     DBuf<int> faces, vf_ptr, vf, rowptr, colind, cptr, contrib, iperm;
     // tree
     DBuf<int> node_start, node_s, node_b, bidx, cmap, child_ptr, child_list, level_nodes, cown, ctail;
@@ -162,14 +167,16 @@ struct fcw_ctx {
         md.m = m;
         md.V = V.p;
         md.faces = faces.p;
-        md.area = area.p;
-        md.gb = gb.p;
         md.larea = larea.p;
         md.g11 = g11.p;
         md.g12 = g12.p;
         md.g22 = g22.p;
-        md.vf_ptr = vf_ptr.p;
-        md.vf = vf.p;
+        md.mo = m;
+        md.op_faces = robust ? op_faces.p : faces.p;
+        md.op_area = robust ? op_area.p : area.p;
+        md.op_gram = robust ? op_gram.p : gram.p;
+        md.op_vf_ptr = vf_ptr.p;
+        md.op_vf = vf.p;
         return md;
     }
 
@@ -487,6 +494,27 @@ double mean_edge_length(const fcw::Symbolic& S, const double* Vpos) {
     return cnt ? sum / (double)cnt : 0.0;
 }
 
+// Mean length of the surface's distinct edges (distance_engines.py:17-21).
+double mean_edge_faces(const double* V, int m, const int* F) {
+    std::vector<uint64_t> e;
+    e.reserve(3 * (size_t)m);
+    for (int f = 0; f < m; ++f)
+        for (int c = 0; c < 3; ++c) {
+            uint32_t a = (uint32_t)F[3 * (size_t)f + c], b = (uint32_t)F[3 * (size_t)f + (c + 1) % 3];
+            if (a > b) std::swap(a, b);
+            e.push_back(((uint64_t)a << 32) | b);
+        }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    double sum = 0.0;
+    for (uint64_t k : e) {
+        const double* p = V + 3 * (size_t)(k >> 32);
+        const double* q = V + 3 * (size_t)(k & 0xffffffffu);
+        sum += std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]));
+    }
+    return e.empty() ? 0.0 : sum / (double)e.size();
+}
+
 void check_faces(int n, const double* V, int m, const int* F) {
     if (n <= 0 || m <= 0) throw Fail("empty mesh");
     for (int64_t i = 0; i < 3LL * m; ++i)
@@ -537,6 +565,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->device = 0;
     opt->ir_steps = 1;
     opt->streams = 8;
+    opt->robust = 0;
     opt->sparse_rhs = 1;
 }
 
@@ -582,7 +611,16 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         double t0 = now_s();
         std::string err;
         fcw::Symbolic& S = C->sym;
-        if (fcw::analyse(n, Vin, m, F, opt.leaf_size > 0 ? opt.leaf_size : 64, S, err)) throw Fail(err);
+        // robust: the operators live on the intrinsic Delaunay triangulation of the mollified surface
+        // (same vertices, flipped faces); the metric kernels keep the surface's own faces
+        fcw::Intrinsic intr;
+        const int* Fop = F;
+        C->robust = opt.robust != 0;
+        if (C->robust) {
+            if (fcw::intrinsic_delaunay(n, Vin, m, F, fcw::kMollify, intr, err)) throw Fail(err);
+            Fop = intr.faces.data();
+        }
+        if (fcw::analyse(n, Vin, m, Fop, opt.leaf_size > 0 ? opt.leaf_size : 64, S, err)) throw Fail(err);
         C->node_of_pos.assign(n, -1);
         for (int id = 0; id < S.nnodes; ++id)
             for (int p = S.node_start[id]; p < S.node_start[id] + S.node_s[id]; ++p) C->node_of_pos[p] = id;
@@ -590,7 +628,22 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         std::vector<double> Vp((size_t)n * 3);
         for (int p = 0; p < n; ++p)
             for (int c = 0; c < 3; ++c) Vp[3 * (size_t)p + c] = Vin[3 * (size_t)S.perm[p] + c];
-        const double mean_edge = mean_edge_length(S, Vp.data());
+        const double mean_edge = C->robust ? mean_edge_faces(Vin, m, F) : mean_edge_length(S, Vp.data());
+        // the surface's faces in positions, ordered by their smallest position (S.faces when the
+        // operators use the same triangulation)
+        std::vector<int> gfaces;
+        if (C->robust) {
+            std::vector<int> idx(m);
+            std::iota(idx.begin(), idx.end(), 0);
+            auto lo = [&](int f) {
+                return std::min({S.iperm[F[3 * (size_t)f]], S.iperm[F[3 * (size_t)f + 1]], S.iperm[F[3 * (size_t)f + 2]]});
+            };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return lo(a) < lo(b); });
+            gfaces.resize(3 * (size_t)m);
+            for (int f = 0; f < m; ++f)
+                for (int a = 0; a < 3; ++a) gfaces[3 * (size_t)f + a] = S.iperm[F[3 * (size_t)idx[f] + a]];
+        }
+        const std::vector<int>& G = C->robust ? gfaces : S.faces;
         const double t = (opt.t > 0.0) ? opt.t : opt.t_coef * mean_edge * mean_edge;
         if (!(t > 0.0) || !std::isfinite(t)) throw Fail("diffusion time must be positive and finite");
         int64_t& tb = C->dev_bytes;
@@ -600,7 +653,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         }
         hipStream_t st = C->ws[0]->stream;
         C->V.upload(Vp.data(), Vp.size(), tb);
-        C->faces.upload(S.faces.data(), S.faces.size(), tb);
+        C->faces.upload(G.data(), G.size(), tb);
         C->vf_ptr.upload(S.vf_ptr.data(), S.vf_ptr.size(), tb);
         C->vf.upload(S.vf.data(), S.vf.size(), tb);
         C->rowptr.upload(S.rowptr.data(), S.rowptr.size(), tb);
@@ -609,7 +662,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->contrib.upload(S.contrib.data(), S.contrib.size(), tb);
         C->iperm.upload(S.iperm.data(), S.iperm.size(), tb);
         C->area.alloc(m, tb);
-        C->gb.alloc((size_t)m * 9, tb);
+        C->gram.alloc((size_t)m * 6, tb);
         C->larea.alloc(m, tb);
         C->g11.alloc(m, tb);
         C->g12.alloc(m, tb);
@@ -617,10 +670,20 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         const int nnz = S.rowptr[n];
         C->lval.alloc(nnz, tb);
         C->mass.alloc(n, tb);
-        fcw::launch_face_geometry(st, m, C->V.p, C->faces.p, C->area.p, C->gb.p, C->larea.p, C->g11.p, C->g12.p,
+        fcw::launch_face_geometry(st, m, C->V.p, C->faces.p, C->area.p, C->gram.p, C->larea.p, C->g11.p, C->g12.p,
                                   C->g22.p);
-        fcw::launch_assemble(st, n, nnz, C->cptr.p, C->contrib.p, C->area.p, C->gb.p, C->vf_ptr.p, C->vf.p,
-                             C->lval.p, C->mass.p);
+        if (C->robust) {
+            // operator faces in the analysis's order; corner c keeps its input corner's opposite edge
+            std::vector<double> len(3 * (size_t)m), oa(m), og(6 * (size_t)m);
+            for (int f = 0; f < m; ++f)
+                for (int c = 0; c < 3; ++c) len[3 * (size_t)f + c] = intr.len[3 * (size_t)S.face_perm[f] + c];
+            fcw::intrinsic_operators(m, len.data(), oa.data(), og.data());
+            C->op_faces.upload(S.faces.data(), S.faces.size(), tb);
+            C->op_area.upload(oa.data(), oa.size(), tb);
+            C->op_gram.upload(og.data(), og.size(), tb);
+        }
+        fcw::launch_assemble(st, n, nnz, C->cptr.p, C->contrib.p, C->robust ? C->op_area.p : C->area.p,
+                             C->robust ? C->op_gram.p : C->gram.p, C->vf_ptr.p, C->vf.p, C->lval.p, C->mass.p);
         {
             // 256-face chunks of the face order: their vertices, each vertex owned by the first
             // chunk that holds it (a vertex in no face by the last chunk), owned vertices first
@@ -629,7 +692,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
             std::vector<int> owner(n, -1), mark(n, -1);
             for (int f = 0; f < m; ++f)
                 for (int a = 0; a < 3; ++a) {
-                    const int p = S.faces[3 * (size_t)f + a];
+                    const int p = G[3 * (size_t)f + a];
                     if (owner[p] < 0) owner[p] = f / kChunk;
                 }
             std::vector<std::vector<int>> owned(std::max(1, nch));
@@ -640,7 +703,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
                 for (int p : owned[c]) mark[p] = c;
                 for (int f = c * kChunk; f < std::min(m, (c + 1) * kChunk); ++f)
                     for (int a = 0; a < 3; ++a) {
-                        const int p = S.faces[3 * (size_t)f + a];
+                        const int p = G[3 * (size_t)f + a];
                         if (mark[p] == c) continue;
                         mark[p] = c;
                         other.push_back(p);
@@ -721,6 +784,10 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         I.factor_flops = S.factor_flops;
         I.solve_flops_per_rhs = S.solve_flops_per_rhs;
         I.t = t;
+        I.robust = C->robust ? 1 : 0;
+        I.intrinsic_flips = intr.flips;
+        I.intrinsic_skipped = intr.skipped;
+        I.mollify_delta = intr.delta;
         I.mean_edge = mean_edge;
         I.setup_seconds[0] = t1 - t0;
         I.setup_seconds[1] = t2 - t1;
@@ -942,6 +1009,25 @@ int fcw_calibration_copy(int64_t bytes, int device) {
         (void)hipFree(a);
         (void)hipFree(b);
         HIPCHK(e);
+    });
+}
+
+int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double mollify, int* faces_out, double* len_out,
+                       double* area_out, double* gram_out, double* stats) {
+    return guard([&] {
+        if (!V || !F || !faces_out || !len_out) throw Fail("null argument");
+        check_faces(n, V, m, F);
+        fcw::Intrinsic I;
+        std::string err;
+        if (fcw::intrinsic_delaunay(n, V, m, F, mollify, I, err)) throw Fail(err);
+        std::copy(I.faces.begin(), I.faces.end(), faces_out);
+        std::copy(I.len.begin(), I.len.end(), len_out);
+        if (area_out && gram_out) fcw::intrinsic_operators(m, I.len.data(), area_out, gram_out);
+        if (stats) {
+            stats[0] = I.flips;
+            stats[1] = I.skipped;
+            stats[2] = I.delta;
+        }
     });
 }
 

@@ -49,15 +49,20 @@ enum ScalarSlot {
 struct MeshDev {
     int n, m;
     const double* V;        // n x 3, positions
+    // the surface's faces (metric kernels)
     const int* faces;       // m x 3, positions
-    const double* area;     // m (cross-product area)
-    const double* gb;       // m x 9 (grad B0, grad B1, grad B2)
-    const double* larea;    // m (Gram area, metric kernels)
+    const double* larea;    // m (Gram area)
     const double* g11;
     const double* g12;
     const double* g22;
-    const int* vf_ptr;      // n + 1
-    const int* vf;          // face*3 + corner
+    // the operator triangulation (L, M, divergence): the surface's own faces, or its intrinsic
+    // Delaunay triangulation (same vertices, other faces)
+    int mo;
+    const int* op_faces;    // mo x 3, positions
+    const double* op_area;  // mo
+    const double* op_gram;  // mo x 6: grad B_a . grad B_b, stored G00 G01 G02 G11 G12 G22
+    const int* op_vf_ptr;   // n + 1
+    const int* op_vf;       // face*3 + corner
 };
 
 struct MetricsDev {
@@ -99,10 +104,10 @@ struct MetricsOut {
     float* samp_a;     // k x (S + (S-1)q)
 };
 
-void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gb,
+void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gram,
                           double* larea, double* g11, double* g12, double* g22);
 void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, const int* contrib, const double* area,
-                     const double* gb, const int* vf_ptr, const int* vf, double* lval, double* mass);
+                     const double* gram, const int* vf_ptr, const int* vf, double* lval, double* mass);
 void launch_fill(hipStream_t st, double* p, int64_t count, double value);
 void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes);
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);

@@ -56,6 +56,21 @@ struct Symbolic {
 
 int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err);
 
+// Intrinsic Delaunay triangulation of the surface (V, F) after mollification with
+// eps = mollify * mean edge length (intrinsic.cpp): same vertices, flipped faces, and the length
+// of the edge opposite each corner.
+struct Intrinsic {
+    std::vector<int> faces;   // m x 3
+    std::vector<double> len;  // m x 3: edge opposite corner c
+    int flips = 0;
+    int skipped = 0;          // non-Delaunay edges whose flip would fold a face
+    double delta = 0.0;       // added to every edge length
+};
+constexpr double kMollify = 1e-6;
+int intrinsic_delaunay(int n, const double* V, int m, const int* F, double mollify, Intrinsic& out, std::string& err);
+// Area and basis-gradient Gram matrix (G00 G01 G02 G11 G12 G22) of each face from its lengths.
+void intrinsic_operators(int m, const double* len, double* area, double* gram);
+
 // Multifrontal FP64 Cholesky of A = diag_scale*Mdiag + off_scale*L (pattern of S, values in CSR
 // order `lval`, lumped mass `mdiag` in positions). Writes the partitioned-inverse panels
 // P_S = [L11^{-1}; L21 L11^{-1}] into `panel` (S.panel_elems doubles).

@@ -17,7 +17,7 @@ namespace {
 // ---------------------------------------------------------------------------
 
 __global__ void face_geometry_kernel(int m, const double* __restrict__ V, const int* __restrict__ faces,
-                                     double* __restrict__ area, double* __restrict__ gb,
+                                     double* __restrict__ area, double* __restrict__ gram,
                                      double* __restrict__ larea, double* __restrict__ g11,
                                      double* __restrict__ g12, double* __restrict__ g22) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -39,12 +39,17 @@ __global__ void face_geometry_kernel(int m, const double* __restrict__ V, const 
     double b2[3] = {(nh[1] * e1[2] - nh[2] * e1[1]) * inv2a, (nh[2] * e1[0] - nh[0] * e1[2]) * inv2a,
                     (nh[0] * e1[1] - nh[1] * e1[0]) * inv2a};
     area[f] = A;
-    double* g = gb + 9 * (int64_t)f;
+    // Gram matrix of the basis gradients grad B0 = -b1 - b2, grad B1 = b1, grad B2 = b2
+    double g[3][3];
     for (int c = 0; c < 3; ++c) {
-        g[c] = -b1[c] - b2[c];
-        g[3 + c] = b1[c];
-        g[6 + c] = b2[c];
+        g[0][c] = -b1[c] - b2[c];
+        g[1][c] = b1[c];
+        g[2][c] = b2[c];
     }
+    double* G = gram + 6 * (int64_t)f;
+    int q = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) G[q++] = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
     const double a11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
     const double a22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
     const double a12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
@@ -57,8 +62,16 @@ __global__ void face_geometry_kernel(int m, const double* __restrict__ V, const 
 }
 
 // One thread per CSR entry: L_e = sum over (face, a, b) of A_f gradB_a . gradB_b (deterministic order).
+// Entry (a, b) of a face's symmetric 3 x 3 Gram matrix, stored G00 G01 G02 G11 G12 G22.
+__device__ inline int gram_index(int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return (lo * (5 - lo)) / 2 + hi;
+}
+
+// L = sum over faces of A_f grad B_a . grad B_b (fastcw_lean.py:394-413), one CSR entry per thread,
+// its (face, corner pair) contributions in a fixed order.
 __global__ void assemble_l_kernel(int nnz, const int* __restrict__ cptr, const int* __restrict__ contrib,
-                                  const double* __restrict__ area, const double* __restrict__ gb,
+                                  const double* __restrict__ area, const double* __restrict__ gram,
                                   double* __restrict__ lval) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
@@ -66,8 +79,7 @@ __global__ void assemble_l_kernel(int nnz, const int* __restrict__ cptr, const i
     for (int c = cptr[e]; c < cptr[e + 1]; ++c) {
         const int code = contrib[c];
         const int f = code / 9, ab = code - 9 * f, a = ab / 3, b = ab - 3 * a;
-        const double* g = gb + 9 * (int64_t)f;
-        s += area[f] * (g[3 * a] * g[3 * b] + g[3 * a + 1] * g[3 * b + 1] + g[3 * a + 2] * g[3 * b + 2]);
+        s += area[f] * gram[6 * (int64_t)f + gram_index(a, b)];
     }
     lval[e] = s;
 }
@@ -121,8 +133,9 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 //
 // A workgroup owns kDivRows consecutive vertex rows and every column of the RHS block. It first
 // stages, cooperatively and in one pass, every incidence (face f, corner a) of its rows in LDS:
-// the byte offsets of the face's three vertex rows and five coefficients from the face's
-// gradient basis (g1 = grad B1, g2 = grad B2, ga = grad Ba) and area A:
+// the byte offsets of the face's three vertex rows and five coefficients from the Gram matrix of
+// the face's gradient basis (g1 = grad B1, g2 = grad B2, ga = grad Ba) and its area A (extrinsic
+// or intrinsic geometry alike):
 //     |grad u|^2        = du1^2 |g1|^2 + du1 du2 (2 g1.g2) + du2^2 |g2|^2
 //     A (grad u . ga)   = du1 (A g1.ga) + du2 (A g2.ga)
 // with du1 = u1 - u0, du2 = u2 - u0, so the per-column work is two differences, two short dot
@@ -160,7 +173,7 @@ template <bool BUF>
 __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, int run,
                                                          const int* __restrict__ vf_ptr, const int* __restrict__ vf,
                                                          const int* __restrict__ faces, const double* __restrict__ area,
-                                                         const double* __restrict__ gb, const double* __restrict__ u,
+                                                         const double* __restrict__ gram, const double* __restrict__ u,
                                                          double* __restrict__ out) {
     __shared__ uint4 rec[kDivInc];       // byte offsets of the face's vertex rows 0, 1, 2
     __shared__ double geo[kDivInc][5];   // A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2
@@ -186,16 +199,19 @@ __global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngr
             const int code = vf[base + tid];
             const int f = code / 3, a = code - 3 * f;
             const int i0 = faces[3 * (int64_t)f], i1 = faces[3 * (int64_t)f + 1], i2 = faces[3 * (int64_t)f + 2];
-            const double* g = gb + 9 * (int64_t)f;
+            const double* Gp = gram + 6 * (int64_t)f;  // G00 G01 G02 G11 G12 G22
+            double G[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) G[c] = Gp[c];
             const double A = area[f];
-            const double ax = g[3 * a], ay = g[3 * a + 1], az = g[3 * a + 2];
-            const double g1x = g[3], g1y = g[4], g1z = g[5], g2x = g[6], g2y = g[7], g2z = g[8];
+            const double g1a = a == 0 ? G[1] : (a == 1 ? G[3] : G[4]);
+            const double g2a = a == 0 ? G[2] : (a == 1 ? G[4] : G[5]);
             rec[tid] = make_uint4((unsigned)i0 * row_bytes, (unsigned)i1 * row_bytes, (unsigned)i2 * row_bytes, 0u);
-            geo[tid][0] = A * (g1x * ax + g1y * ay + g1z * az);
-            geo[tid][1] = A * (g2x * ax + g2y * ay + g2z * az);
-            geo[tid][2] = g1x * g1x + g1y * g1y + g1z * g1z;
-            geo[tid][3] = 2.0 * (g1x * g2x + g1y * g2y + g1z * g2z);
-            geo[tid][4] = g2x * g2x + g2y * g2y + g2z * g2z;
+            geo[tid][0] = A * g1a;
+            geo[tid][1] = A * g2a;
+            geo[tid][2] = G[3];
+            geo[tid][3] = 2.0 * G[4];
+            geo[tid][4] = G[5];
         }
         __syncthreads();
         for (int pr = w; pr < npairs; pr += 4) {
@@ -301,14 +317,14 @@ inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)
 
 }  // namespace
 
-void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gb,
+void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gram,
                           double* larea, double* g11, double* g12, double* g22) {
-    face_geometry_kernel<<<grid1(m, 256), 256, 0, st>>>(m, V, faces, area, gb, larea, g11, g12, g22);
+    face_geometry_kernel<<<grid1(m, 256), 256, 0, st>>>(m, V, faces, area, gram, larea, g11, g12, g22);
 }
 
 void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, const int* contrib, const double* area,
-                     const double* gb, const int* vf_ptr, const int* vf, double* lval, double* mass) {
-    assemble_l_kernel<<<grid1(nnz, 256), 256, 0, st>>>(nnz, contrib_ptr, contrib, area, gb, lval);
+                     const double* gram, const int* vf_ptr, const int* vf, double* lval, double* mass) {
+    assemble_l_kernel<<<grid1(nnz, 256), 256, 0, st>>>(nnz, contrib_ptr, contrib, area, gram, lval);
     lumped_mass_kernel<<<grid1(n, 256), 256, 0, st>>>(n, vf_ptr, vf, area, mass);
 }
 
@@ -340,11 +356,11 @@ void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const doubl
     const int run = (ngroups + kXcds - 1) / kXcds;  // row groups per XCD
     const uint64_t bytes = (uint64_t)mesh.n * (uint64_t)ldk * 8u;  // below 4 GiB (checked at creation)
     if (bytes < (1ull << 31))
-        divergence_kernel<true><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf,
-                                                                   mesh.faces, mesh.area, mesh.gb, u, out);
+        divergence_kernel<true><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.op_vf_ptr, mesh.op_vf,
+                                                                   mesh.op_faces, mesh.op_area, mesh.op_gram, u, out);
     else
-        divergence_kernel<false><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.vf_ptr, mesh.vf,
-                                                                    mesh.faces, mesh.area, mesh.gb, u, out);
+        divergence_kernel<false><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.op_vf_ptr, mesh.op_vf,
+                                                                    mesh.op_faces, mesh.op_area, mesh.op_gram, u, out);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,

@@ -21,6 +21,12 @@ import numpy as np
 from . import mesh_prep
 
 ENGINE_NAME = "mi355x_heat"
+# Same backend behind the reference's production-engine contract (PotpourriDistanceEngine,
+# distance_engines.py:72-125): robust operators always, t = diffusion_length_mm^2.
+POTPOURRI_NAME = "mi355x_potpourri"
+DEFAULT_DIFFUSION_LENGTH_MM = 0.7         # distance_engines.py:12-14
+MIN_DIFFUSION_LENGTH_MM = 0.5
+RECOMMENDED_MIN_DIFFUSION_LENGTH_MM = 0.7
 
 
 class BaseDistanceEngine(ABC):
@@ -41,8 +47,9 @@ class BaseDistanceEngine(ABC):
         raise NotImplementedError
 
 
-def _backend_mixin(base):
-    """The heat backend body, parameterised on the base class it must derive from."""
+def _backend_mixin(base, potpourri=False):
+    """The heat backend body, parameterised on the base class it must derive from. potpourri:
+    the kwargs contract of the reference's PotpourriDistanceEngine (distance_engines.py:75-89)."""
 
     class _Hip(base):
         """MI355X heat-method distances (fastcw_lean.py:467-534 operators) behind compute_distance.
@@ -52,21 +59,40 @@ def _backend_mixin(base):
           diffusion_length_mm (float): t = L**2 (the potpourri convention, distance_engines.py:119-124);
           poisson_shift (1e-10), block (256 sources per solve), precision ('fp64' | 'fp32_ir'),
           ir_steps (1, refinement steps of 'fp32_ir'), leaf_size (64), device (0), host_threads (0),
-          streams (1: this adapter solves one block per miss, so more workspaces would sit idle).
+          streams (1: this adapter solves one block per miss, so more workspaces would sit idle),
+          use_robust (False: operators on the surface's own triangulation, as ConsistentBatchHeatEngine;
+          True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode).
         Sources are solved in RHS blocks: a miss solves the next ``block`` sources of the planned
         order (default: the reference's BFS order, core_analysis.py:1384-1417), so the caller's
         one-source-at-a-time loop reads from the cache.
         """
 
         _ALLOWED = {"t_coef", "diffusion_length_mm", "poisson_shift", "block", "precision", "leaf_size", "device",
-                    "ir_steps", "host_threads", "streams"}
+                    "ir_steps", "host_threads", "streams", "use_robust"} | ({"allow_eigen_fallback"} if potpourri else set())
+        _NAME = POTPOURRI_NAME if potpourri else ENGINE_NAME
 
         def __init__(self, vertices, faces, **engine_kwargs):
             super().__init__(vertices, faces, **engine_kwargs)
             kw = dict(self.engine_kwargs)
             unknown = sorted(set(kw) - self._ALLOWED)
             if unknown:
-                raise ValueError(f"Unsupported engine kwargs for {ENGINE_NAME} engine: {', '.join(unknown)}")
+                raise ValueError(f"Unsupported engine kwargs for {self._NAME} engine: {', '.join(unknown)}")
+            if potpourri:
+                kw.pop("allow_eigen_fallback", None)  # no Eigen/SuiteSparse backends to choose between
+                if "use_robust" in kw and not bool(kw.pop("use_robust")):
+                    raise ValueError("use_robust=False is not supported; robust mode is always used.")
+                if "t_coef" in kw:
+                    raise ValueError("Set diffusion_length_mm instead of t_coef.")
+                L = float(kw.pop("diffusion_length_mm", DEFAULT_DIFFUSION_LENGTH_MM))
+                if not np.isfinite(L) or L < MIN_DIFFUSION_LENGTH_MM:
+                    raise ValueError(f"diffusion_length_mm must be finite and at least {MIN_DIFFUSION_LENGTH_MM} mm.")
+                if L < RECOMMENDED_MIN_DIFFUSION_LENGTH_MM:
+                    warnings.warn(f"diffusion_length_mm={L} is below the validated {RECOMMENDED_MIN_DIFFUSION_LENGTH_MM} mm.",
+                                  RuntimeWarning)
+                kw["diffusion_length_mm"] = L
+                robust = True
+            else:
+                robust = bool(kw.pop("use_robust", False))
             if "t_coef" in kw and "diffusion_length_mm" in kw:
                 raise ValueError("Set either t_coef or diffusion_length_mm, not both.")
             self.mean_edge_length = mesh_prep.mean_edge_length(self.vertices, self.faces)
@@ -90,23 +116,23 @@ def _backend_mixin(base):
             precision = str(kw.pop("precision", "fp64")).lower()
             if precision not in ("fp64", "fp32_ir"):
                 raise ValueError("precision must be 'fp64' or 'fp32_ir'.")
-            self.use_robust = False
+            self.use_robust = robust
             try:
                 from . import _lib
                 from .engine import HeatContext
 
                 _lib.lib()
             except ImportError as exc:
-                raise ImportError(f"libfcw.so is required for the '{ENGINE_NAME}' engine.") from exc
+                raise ImportError(f"libfcw.so is required for the '{self._NAME}' engine.") from exc
             info = _lib.build_info()
             if "gfx950" not in info:
-                raise RuntimeError(f"{ENGINE_NAME} backend self-check failed: {info!r}")
+                raise RuntimeError(f"{self._NAME} backend self-check failed: {info!r}")
             self.backend_info = info
             self._ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t, poisson_shift=shift,
                                     block=int(kw.pop("block", 256)), leaf_size=int(kw.pop("leaf_size", 64)),
                                     precision=precision, ir_steps=int(kw.pop("ir_steps", 1)),
                                     device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)),
-                                    streams=int(kw.pop("streams", 1)))
+                                    streams=int(kw.pop("streams", 1)), robust=robust)
             self.t = self._ctx.t
             self._plan = None
             self._plan_pos = None
@@ -114,7 +140,7 @@ def _backend_mixin(base):
 
         @property
         def name(self) -> str:
-            return ENGINE_NAME
+            return self._NAME
 
         @property
         def context(self):
@@ -155,9 +181,13 @@ def _backend_mixin(base):
 MI355XHeatDistanceEngine = _backend_mixin(BaseDistanceEngine)
 MI355XHeatDistanceEngine.__name__ = "MI355XHeatDistanceEngine"
 MI355XHeatDistanceEngine.__qualname__ = "MI355XHeatDistanceEngine"
+MI355XPotpourriDistanceEngine = _backend_mixin(BaseDistanceEngine, potpourri=True)
+MI355XPotpourriDistanceEngine.__name__ = "MI355XPotpourriDistanceEngine"
+MI355XPotpourriDistanceEngine.__qualname__ = "MI355XPotpourriDistanceEngine"
 
 ENGINE_REGISTRY = {
     ENGINE_NAME: MI355XHeatDistanceEngine,
+    POTPOURRI_NAME: MI355XPotpourriDistanceEngine,
 }
 
 
@@ -171,12 +201,18 @@ def create_distance_engine(engine_type, vertices, faces, engine_kwargs=None):
     return cls(vertices, faces, **dict(engine_kwargs or {}))
 
 
-def register_into(module, name=ENGINE_NAME):
-    """Register the backend in another distance_engines module, subclassing *its* BaseDistanceEngine."""
+def register_into(module, name=ENGINE_NAME, potpourri_name=POTPOURRI_NAME):
+    """Register the backend in another distance_engines module, subclassing *its*
+    BaseDistanceEngine: ``name`` with this module's kwargs, ``potpourri_name`` (unless None) with
+    the reference's potpourri-engine contract. Returns the ``name`` class."""
     base = getattr(module, "BaseDistanceEngine")
     cls = _backend_mixin(base)
     cls.__name__ = cls.__qualname__ = "MI355XHeatDistanceEngine"
     module.ENGINE_REGISTRY[name] = cls
+    if potpourri_name:
+        pcls = _backend_mixin(base, potpourri=True)
+        pcls.__name__ = pcls.__qualname__ = "MI355XPotpourriDistanceEngine"
+        module.ENGINE_REGISTRY[potpourri_name] = pcls
     if not issubclass(cls, base):  # pragma: no cover - defensive
         warnings.warn("registered backend does not derive from the module's BaseDistanceEngine")
     return cls

@@ -19,11 +19,13 @@ class HeatContext:
     """Heat-method geodesics + wiring metrics on one mesh (fastcw_lean.py:467-534 counterpart).
 
     Parameters mirror ``ConsistentBatchHeatEngine(vertices, faces, t_coef, poisson_shift)``
-    (fastcw_lean.py:481); ``t`` overrides ``t_coef * mean_edge**2``.
+    (fastcw_lean.py:481); ``t`` overrides ``t_coef * mean_edge**2``. ``robust=True`` builds the
+    operators on the intrinsic Delaunay triangulation of the mollified surface, as potpourri3d's
+    ``use_robust=True`` (distance_engines.py:78-79, 119-125).
     """
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
-                 precision="fp64", ir_steps=1, device=0, host_threads=0, streams=8, sparse_rhs=True):
+                 precision="fp64", ir_steps=1, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
         if V.ndim != 2 or V.shape[1] != 3:
@@ -44,6 +46,7 @@ class HeatContext:
         opt.ir_steps = int(ir_steps)
         opt.streams = int(streams)
         opt.sparse_rhs = 1 if sparse_rhs else 0
+        opt.robust = 1 if robust else 0
         handle = ctypes.c_void_p()
         _lib.check(L.fcw_create(_lib.dptr(V), V.shape[0], _lib.iptr(F), F.shape[0], ctypes.byref(opt),
                                 ctypes.byref(handle)))

@@ -55,7 +55,9 @@ typedef struct {
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces), 1..16;
                               default 8 (run concurrently when GPU_MAX_HW_QUEUES >= streams) */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
-    int reserved[4];
+    int robust;            /* 1: operators on the intrinsic Delaunay triangulation of the mollified surface,
+                              potpourri3d's use_robust=True (distance_engines.py:78-79, 119-125); default 0 */
+    int reserved[3];
 } fcw_options;
 
 typedef struct {
@@ -67,6 +69,10 @@ typedef struct {
     double t, mean_edge;
     double setup_seconds[4];    /* analyse, assemble, factor (both systems), upload */
     int64_t device_bytes;       /* device memory held by the context */
+    int robust;                 /* operators on the intrinsic Delaunay triangulation */
+    int intrinsic_flips;        /* edge flips made (robust) */
+    int intrinsic_skipped;      /* non-Delaunay edges left because flipping them would fold a face */
+    double mollify_delta;       /* length added to every edge by the mollification (robust) */
 } fcw_info;
 
 typedef struct {
@@ -142,6 +148,14 @@ int fcw_calibration_copy(int64_t bytes, int device);
  * nnodes, nlevels, factor_nnz, panel_elems. */
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale,
                           int leaf_size, const double* B, int k, double* X, double* stats);
+
+/* Host-only: the intrinsic Delaunay triangulation the robust operators are built on (the
+ * triangulation inside potpourri3d's robust mode, distance_engines.py:119-125, which has no
+ * callable counterpart). faces_out m x 3 (same vertices, flipped connectivity), len_out m x 3
+ * (length of the edge opposite each corner), area_out m, gram_out m x 6 (grad B_a . grad B_b:
+ * G00 G01 G02 G11 G12 G22). stats[0..2] = flips, skipped flips, mollification delta. */
+int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double mollify, int* faces_out, double* len_out,
+                       double* area_out, double* gram_out, double* stats);
 
 #ifdef __cplusplus
 }

@@ -153,7 +153,14 @@ def test_engine_kwargs_rules_raise_before_any_gpu_work(surf_c1):
 
     V, F = surf_c1.vertices, surf_c1.faces
     with pytest.raises(ValueError, match="Unsupported"):
-        de.create_distance_engine("mi355x_heat", V, F, {"use_robust": True})
+        de.create_distance_engine("mi355x_heat", V, F, {"allow_eigen_fallback": True})
+    # the potpourri contract (distance_engines.py:78-89): robust only, diffusion_length_mm >= 0.5
+    with pytest.raises(ValueError, match="use_robust=False"):
+        de.create_distance_engine("mi355x_potpourri", V, F, {"use_robust": False})
+    with pytest.raises(ValueError, match="diffusion_length_mm instead"):
+        de.create_distance_engine("mi355x_potpourri", V, F, {"t_coef": 1.0})
+    with pytest.raises(ValueError, match="at least 0.5"):
+        de.create_distance_engine("mi355x_potpourri", V, F, {"diffusion_length_mm": 0.4})
     with pytest.raises(ValueError, match="either"):
         de.create_distance_engine("mi355x_heat", V, F, {"t_coef": 1.0, "diffusion_length_mm": 0.7})
     with pytest.raises(ValueError):
@@ -192,6 +199,7 @@ def test_register_into_foreign_module():
     mod.ENGINE_REGISTRY = {}
     cls = de.register_into(mod)
     assert mod.ENGINE_REGISTRY["mi355x_heat"] is cls and issubclass(cls, Base)
+    assert issubclass(mod.ENGINE_REGISTRY["mi355x_potpourri"], Base)
 
 
 def test_bench_refuses_multi_gpu():
