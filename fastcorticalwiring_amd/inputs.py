@@ -24,7 +24,8 @@ import zlib
 import numpy as np
 
 __all__ = ["read_freesurfer_surface", "write_freesurfer_surface", "read_freesurfer_label",
-           "read_freesurfer_annot", "read_gifti_arrays", "load_surface_and_mask", "infer_output_basename"]
+           "read_freesurfer_annot", "read_gifti_arrays", "load_surface_and_mask", "infer_output_basename",
+           "select_vertex_subset", "random_vertex_sampling", "stratified_vertex_sampling", "farthest_point_sampling"]
 
 FS_TRIANGLE_MAGIC = 0xFFFFFE
 
@@ -328,3 +329,98 @@ def load_surface_and_mask(*, standard, surface_path=None, mask_path=None, subjec
             meta["mask_source"] = f"gifti_mask:{mask_path}"
         return V, F, mask.astype(bool), meta
     raise ValueError(f"Unsupported standard '{standard}'. Expected 'freesurfer' or 'fslr'.")
+
+
+# --------------------------------------------------------------------------------------------
+# Vertex subsets (io_utils.py:496-630): the same selections for the same seed
+# --------------------------------------------------------------------------------------------
+def random_vertex_sampling(valid_mask, k, random_state=None):
+    """k cortical vertices uniformly without replacement (all of them when k covers them)."""
+    valid = np.flatnonzero(valid_mask)
+    if k >= valid.size:
+        return valid.tolist()
+    pick = np.random.default_rng(random_state).choice(valid, size=int(k), replace=False)
+    return np.asarray(pick, dtype=np.int64).tolist()
+
+
+def stratified_vertex_sampling(vertices, valid_mask, k, n_bins=None, random_state=None):
+    """k cortical vertices spread over the cortical order: contiguous bins get floor shares of k,
+    the remainder goes to the bins with the largest fractional shares, each bin sampled without
+    replacement (cortical order is a cheap proxy for spatial spread)."""
+    del vertices
+    valid = np.flatnonzero(valid_mask)
+    nv = valid.size
+    if k >= nv:
+        return valid.tolist()
+    k = int(k)
+    nb = max(8, min(256, int(np.sqrt(k)))) if n_bins is None else int(n_bins)
+    nb = max(1, min(nb, nv))
+    bins = np.array_split(valid, nb)
+    size = np.array([b.size for b in bins], dtype=np.int64)
+    share = size / float(nv) * float(k)
+    take = np.minimum(np.floor(share).astype(np.int64), size)
+    left = k - int(take.sum())
+    if left > 0:
+        for i in np.argsort(-(share - take)):
+            if left <= 0:
+                break
+            room = int(size[i] - take[i])
+            if room > 0:
+                t = min(room, left)
+                take[i] += t
+                left -= t
+    rng = np.random.default_rng(random_state)
+    parts = [np.asarray(b, dtype=np.int64) if t >= b.size else np.asarray(rng.choice(b, size=int(t), replace=False), np.int64)
+             for b, t in zip(bins, take) if t > 0]
+    chosen = np.concatenate(parts) if parts else np.empty(0, np.int64)
+    if chosen.size < k:  # top up from what is left (only when rounding left a gap)
+        rest = np.setdiff1d(valid, chosen, assume_unique=False)
+        extra = rng.choice(rest, size=int(k - chosen.size), replace=False)
+        chosen = np.concatenate([chosen, np.asarray(extra, np.int64)])
+    return chosen.tolist()
+
+
+def farthest_point_sampling(vertices, k, valid_mask):
+    """Euclidean farthest-point sampling over the cortical vertices, from the first of them."""
+    valid = np.flatnonzero(valid_mask)
+    if k >= valid.size:
+        return valid.tolist()
+    P = np.asarray(vertices, dtype=np.float64)[valid]
+    out = np.zeros(int(k), dtype=np.int64)
+    dmin = np.full(P.shape[0], np.inf)
+    cur = 0
+    for i in range(int(k)):
+        out[i] = cur
+        dmin = np.minimum(dmin, np.sum((P - P[cur]) ** 2, axis=1))
+        cur = int(np.argmax(dmin))
+    return valid[out].tolist()
+
+
+def select_vertex_subset(vertices, cortex_mask, *, sample_frac=None, sample_count=None, sample_method="stratified",
+                         random_state=None):
+    """Cortical subset by fraction or count with the named method; None when neither is given."""
+    valid = np.flatnonzero(cortex_mask)
+    if valid.size == 0:
+        return []
+    if sample_frac is not None and sample_count is not None:
+        raise ValueError("sample_frac and sample_count are mutually exclusive.")
+    if sample_count is not None:
+        k = int(sample_count)
+        if k <= 0:
+            raise ValueError("sample_count must be a positive integer.")
+    elif sample_frac is not None:
+        frac = float(sample_frac)
+        if frac <= 0.0 or frac > 1.0:
+            raise ValueError("sample_frac must satisfy 0 < sample_frac <= 1.")
+        k = max(1, int(round(frac * valid.size)))
+    else:
+        return None
+    k = min(k, valid.size)
+    method = str(sample_method or "stratified").lower()
+    if method == "stratified":
+        return stratified_vertex_sampling(vertices, cortex_mask, k, random_state=random_state)
+    if method == "random":
+        return random_vertex_sampling(cortex_mask, k, random_state=random_state)
+    if method == "fps":
+        return farthest_point_sampling(vertices, k, cortex_mask)
+    raise ValueError(f"Unknown sample_method: {sample_method}")
