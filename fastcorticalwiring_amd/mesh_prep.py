@@ -76,12 +76,19 @@ def build_cortex_submesh(V, F, mask):
 
 
 def boundary_indices(F):
-    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]], axis=0).astype(np.int32)
-    e.sort(axis=1)
-    ue, cnt = np.unique(e, axis=0, return_counts=True)
-    be = ue[cnt == 1]
-    if be.size:
-        return np.unique(be.reshape(-1)).astype(np.int32)
+    """Sorted vertices on edges that belong to one face (core_analysis.py:651-665); edges keyed as
+    one int64 (lo * n + hi) so the counting is a flat sort."""
+    F = np.asarray(F, dtype=np.int64)
+    if F.size == 0:
+        return np.empty(0, dtype=np.int32)
+    n = int(F.max()) + 1
+    a = np.concatenate([F[:, 0], F[:, 1], F[:, 2]])
+    b = np.concatenate([F[:, 1], F[:, 2], F[:, 0]])
+    key = np.minimum(a, b) * n + np.maximum(a, b)
+    uk, cnt = np.unique(key, return_counts=True)
+    bk = uk[cnt == 1]
+    if bk.size:
+        return np.unique(np.concatenate([bk // n, bk % n])).astype(np.int32)
     return np.empty(0, dtype=np.int32)
 
 
@@ -100,29 +107,30 @@ def adjacency_csr(n, F):
 
 
 def bfs_order_all(n, F, start=0):
+    """Breadth-first order over every component (core_analysis.py:1384-1417): from ``start``,
+    neighbours in ascending order, then from each unvisited vertex in index order. scipy's
+    compiled breadth-first order visits a vertex's neighbours in CSR order, which is ascending
+    here, so the order is the same as a queue over sorted neighbour lists."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import breadth_first_order
+
+    if n == 0:
+        return np.empty(0, dtype=np.int64)
     indptr, cols = adjacency_csr(n, F)
-    nbrs = [cols[indptr[i]:indptr[i + 1]].tolist() for i in range(n)]
-    visited = bytearray(n)
-    order = []
-
-    def bfs_from(s):
-        q = deque([s])
-        visited[s] = 1
-        while q:
-            v = q.popleft()
-            order.append(v)
-            for u in nbrs[v]:
-                if not visited[u]:
-                    visited[u] = 1
-                    q.append(u)
-
+    G = sp.csr_matrix((np.ones(cols.size, dtype=np.int8), cols, indptr), shape=(n, n))
+    G.has_sorted_indices = True
+    visited = np.zeros(n, dtype=bool)
+    parts = []
     s0 = start if 0 <= start < n else 0
-    if n:
-        bfs_from(s0)
-    for v in range(n):
-        if not visited[v]:
-            bfs_from(v)
-    return np.asarray(order, dtype=np.int64)
+    for s in [s0] + list(range(n)):
+        if visited[s]:
+            continue
+        part = breadth_first_order(G, s, directed=True, return_predecessors=False)
+        visited[part] = True
+        parts.append(part)
+        if visited.all():
+            break
+    return np.concatenate(parts).astype(np.int64)
 
 
 def face_geometry(V, F):
