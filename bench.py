@@ -135,6 +135,7 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=8, ch
     t0 = time.perf_counter()
     res = ctx.wiring(src)
     wall = time.perf_counter() - t0
+    issue = ctx.stage_times()["issue"]  # host seconds spent issuing the timed blocks
     flags = int(np.sum(res["scalars"][:, 0] > 0))
     ovf = int(np.sum(res["scalars"][:, 8] > 0))
     # outputs kept for the accuracy report (not timed): all sources' metrics in `src` order, and
@@ -148,7 +149,8 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams=8, ch
     out = dict(precision=precision, sources=int(src.size), steps=int(steps), seconds=wall,
                sources_per_s=src.size / wall, ms_per_step=1e3 * wall / steps, setup_s=setup,
                hemisphere_latency_s=setup + wall * (n_src / src.size),
-               end_to_end_sources_per_s=n_src / (setup + wall * (n_src / src.size)), flagged=flags, band_overflow=ovf,
+               end_to_end_sources_per_s=n_src / (setup + wall * (n_src / src.size)), host_issue_s=issue,
+               flagged=flags, band_overflow=ovf,
                info=ctx.info, outputs=outputs)
     # distance fields handed to the caller's host memory (fcw_distance_block: solve, shift, one
     # device-to-host copy of k x n doubles per block), the adapter's path; PCIe-inclusive, reported
@@ -371,7 +373,8 @@ def main():
         "forms": {k: {"sources_per_s": v["sources_per_s"], "ms_per_step": v["ms_per_step"], "setup_s": v["setup_s"],
                       "hemisphere_latency_s": v["hemisphere_latency_s"],
                       "end_to_end_sources_per_s": v["end_to_end_sources_per_s"],
-                      "distances_to_host_sources_per_s": v["distances_to_host_sources_per_s"], "flagged": v["flagged"]}
+                      "distances_to_host_sources_per_s": v["distances_to_host_sources_per_s"],
+                      "host_issue_s": v["host_issue_s"], "timed_s": v["seconds"], "flagged": v["flagged"]}
                   for k, v in results.items()},
         "hemisphere_latency_s": best["hemisphere_latency_s"],
         "accuracy": accuracy_report,

@@ -118,6 +118,11 @@ struct fcw_ctx {
     DBuf<int> op_faces;
     DBuf<double> op_area, op_gram;
     DBuf<int> faces, vf_ptr, vf, rowptr, colind, cptr, contrib, iperm;
+    // divergence row groups
+    int div_ngroups = 0;
+    DBuf<int> div_grp, div_verts;
+    DBuf<unsigned> div_loc;
+    DBuf<double> div_geo;
     // tree
     DBuf<int> node_start, node_s, node_b, bidx, cmap, child_ptr, child_list, level_nodes, cown, ctail;
     DBuf<int64_t> node_boff, node_poff, node_uoff;
@@ -145,7 +150,7 @@ struct fcw_ctx {
     DBuf<float> o_sr, o_sa;
     size_t o_cap = 0;
     // timing
-    double stage[5] = {0, 0, 0, 0, 0};
+    double stage[6] = {0, 0, 0, 0, 0, 0};
     bool profiling = false;
     double ktime[kNumTimed] = {0, 0, 0};
     int64_t kcount[kNumTimed] = {0, 0, 0};
@@ -178,6 +183,19 @@ struct fcw_ctx {
         md.op_vf_ptr = vf_ptr.p;
         md.op_vf = vf.p;
         return md;
+    }
+
+    fcw::DivDev div_dev() const {
+        fcw::DivDev d;
+        d.ngroups = div_ngroups;
+        d.grp = reinterpret_cast<const int4*>(div_grp.p);
+        d.verts = div_verts.p;
+        d.loc = div_loc.p;
+        d.geo = div_geo.p;
+        d.vf_ptr = vf_ptr.p;
+        d.vf = vf.p;
+        d.faces = robust ? op_faces.p : faces.p;
+        return d;
     }
 
     EventPair* ev_begin(Workspace& w, int which) {
@@ -310,13 +328,20 @@ struct fcw_ctx {
             for (auto& e : ev) HIPCHK(hipEventCreate(&e));
             HIPCHK(hipEventRecord(ev[0], w.stream));
         }
-        fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
-        if (plan)
+        if (plan) {
+            // the sparse forward sweep reads only the own rows of the plan's supernodes: those rows
+            // of X are cleared (X holds the previous block's divergence), not the whole block
+            const int L = sym.nlevels;
+            fcw::launch_clear_rows(w.stream, plan_nodes.p + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
+                                   ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, w.X.p, nullptr);
+            fcw::launch_set_delta(w.stream, w.X.p, ldk, w.src_pos.p, k);
             solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p);
-        else
+        } else {
+            fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
             solve(w, panel_heat.p, nullptr, w.Yp.p, w.Up.p);
+        }
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
-        fcw::launch_divergence(w.stream, mesh_dev(), ldk, w.Z.p, w.X.p);
+        fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p);
         if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
         if (opt.precision == FCW_FP32_IR)
             poisson_refined(w);
@@ -492,6 +517,61 @@ double mean_edge_length(const fcw::Symbolic& S, const double* Vpos) {
             ++cnt;
         }
     return cnt ? sum / (double)cnt : 0.0;
+}
+
+// Row groups of the divergence: consecutive rows added while the group's incidences touch at most
+// kDivVerts distinct vertices and number at most kDivInc, and it has at most kDivGroupRows rows.
+// grp holds four ints per group (first row, first incidence, first entry in verts, distinct count;
+// -1 for a lone row beyond the limits) and a closing entry (n, total incidences, |verts|, 0);
+// loc[e] packs the local indices of incidence e's three face vertices, 6 bits each.
+void build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::vector<int>& vf,
+                             const std::vector<int>& faces, std::vector<int>& grp, std::vector<int>& verts,
+                             std::vector<unsigned>& loc) {
+    std::vector<int> mark(n, -1), seen(n, -1), lidx(n, 0), cur;
+    int r = 0, gid = 0, attempt = 0;  // a row that does not fit is tried again as the next group's first
+    while (r < n) {
+        const int r0 = r, e0 = vf_ptr[r0];
+        cur.clear();
+        while (r < n && r - r0 < fcw::kDivGroupRows) {
+            int add = 0;
+            ++attempt;
+            for (int e = vf_ptr[r]; e < vf_ptr[r + 1]; ++e)
+                for (int c = 0; c < 3; ++c) {
+                    const int p = faces[3 * (size_t)(vf[e] / 3) + c];
+                    if (mark[p] != gid && seen[p] != attempt) {
+                        seen[p] = attempt;
+                        ++add;
+                    }
+                }
+            const bool fits = (int)cur.size() + add <= fcw::kDivVerts && vf_ptr[r + 1] - e0 <= fcw::kDivInc;
+            if (!fits) break;
+            for (int e = vf_ptr[r]; e < vf_ptr[r + 1]; ++e)
+                for (int c = 0; c < 3; ++c) {
+                    const int p = faces[3 * (size_t)(vf[e] / 3) + c];
+                    if (mark[p] != gid) {
+                        mark[p] = gid;
+                        cur.push_back(p);
+                    }
+                }
+            ++r;
+        }
+        if (r == r0) {  // this row alone exceeds the limits
+            grp.insert(grp.end(), {r0, e0, (int)verts.size(), -1});
+            ++r;
+            ++gid;
+            continue;
+        }
+        std::sort(cur.begin(), cur.end());
+        for (size_t i = 0; i < cur.size(); ++i) lidx[cur[i]] = (int)i;
+        for (int e = e0; e < vf_ptr[r]; ++e) {
+            const int* fv = faces.data() + 3 * (size_t)(vf[e] / 3);
+            loc[e] = (unsigned)lidx[fv[0]] | ((unsigned)lidx[fv[1]] << 6) | ((unsigned)lidx[fv[2]] << 12);
+        }
+        grp.insert(grp.end(), {r0, e0, (int)verts.size(), (int)cur.size()});
+        verts.insert(verts.end(), cur.begin(), cur.end());
+        ++gid;
+    }
+    grp.insert(grp.end(), {n, vf_ptr[n], (int)verts.size(), 0});
 }
 
 // Mean length of the surface's distinct edges (distance_engines.py:17-21).
@@ -684,6 +764,19 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         }
         fcw::launch_assemble(st, n, nnz, C->cptr.p, C->contrib.p, C->robust ? C->op_area.p : C->area.p,
                              C->robust ? C->op_gram.p : C->gram.p, C->vf_ptr.p, C->vf.p, C->lval.p, C->mass.p);
+        {
+            // divergence row groups over the operator triangulation's incidences
+            std::vector<int> grp, verts;
+            std::vector<unsigned> loc(std::max<size_t>(1, S.vf.size()), 0u);
+            build_divergence_groups(n, S.vf_ptr, S.vf, S.faces, grp, verts, loc);
+            C->div_ngroups = (int)(grp.size() / 4) - 1;
+            C->div_grp.upload(grp.data(), grp.size(), tb);
+            C->div_verts.upload(verts.data(), std::max<size_t>(1, verts.size()), tb);
+            C->div_loc.upload(loc.data(), loc.size(), tb);
+            C->div_geo.alloc(std::max<size_t>(1, S.vf.size()) * 5, tb);
+            fcw::launch_incidence_geometry(st, (int)S.vf.size(), C->vf.p, C->robust ? C->op_area.p : C->area.p,
+                                           C->robust ? C->op_gram.p : C->gram.p, C->div_geo.p);
+        }
         {
             // 256-face chunks of the face order: their vertices, each vertex owned by the first
             // chunk that holds it (a vertex in no face by the last chunk), owned vertices first
@@ -937,6 +1030,7 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
                 C->ev_collect(w);
             }
         }
+        C->stage[5] = now_s() - w0;
         C->copy_outputs((size_t)nsrc, out);
         for (auto& w : C->ws) C->ev_collect(*w);
         C->stage[4] = now_s() - w0;
@@ -968,7 +1062,7 @@ int fcw_geometry_fields(fcw_ctx* C, const double* D, const int* src, int k, int 
 
 int fcw_stage_times(fcw_ctx* C, double* out, int len) {
     if (!C || !out) return 0;
-    int c = std::min(len, 5);
+    int c = std::min(len, 6);
     for (int i = 0; i < c; ++i) out[i] = C->stage[i];
     return c;
 }
@@ -982,7 +1076,7 @@ int fcw_set_profiling(fcw_ctx* C, int on) {
             C->ktime[i] = 0.0;
             C->kcount[i] = 0;
         }
-        for (int i = 0; i < 5; ++i) C->stage[i] = 0.0;
+        for (int i = 0; i < 6; ++i) C->stage[i] = 0.0;
     });
 }
 

@@ -65,6 +65,23 @@ struct MeshDev {
     const int* op_vf;       // face*3 + corner
 };
 
+// Row groups of the divergence (built once per mesh): consecutive vertex rows whose incidences
+// touch at most kDivVerts distinct vertices, at most kDivInc incidences, at most kDivGroupRows rows.
+constexpr int kDivVerts = 64;
+constexpr int kDivInc = 128;
+constexpr int kDivGroupRows = 64;
+struct DivDev {
+    int ngroups;
+    const int4* grp;       // ngroups + 1: first row, first incidence, first distinct vertex (into verts), count
+                           // (-1: one row beyond the limits, read from global memory); entry ngroups has .x = n
+    const int* verts;      // distinct vertex positions of each group, ascending
+    const unsigned* loc;   // per incidence: local indices of the face's vertices 0, 1, 2 (6 bits each)
+    const double* geo;     // per incidence: A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2
+    const int* vf_ptr;     // n + 1, the operator triangulation's incidences per row
+    const int* vf;         // face*3 + corner
+    const int* faces;      // operator faces, positions
+};
+
 struct MetricsDev {
     int S, q, nb;
     const double* targets;  // S (ascending)
@@ -111,6 +128,9 @@ void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, con
 void launch_fill(hipStream_t st, double* p, int64_t count, double value);
 void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes);
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k);
+// X[src_pos[j]][j] = 1 only (the caller has zeroed every row the sweep reads).
+void launch_set_delta(hipStream_t st, double* X, int ldk, const int* src_pos, int k);
+// Zero the own rows (in Y) and, with U set, the update rows of the listed supernodes.
 void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
                        const int* node_s, const int* node_b, const int64_t* node_uoff, double* Y, double* U);
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -139,7 +159,10 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r);
 void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count);
-void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out);
+// Per-incidence coefficients of the divergence, computed once per mesh (see DivDev::geo).
+void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const double* area, const double* gram,
+                               double* geo);
+void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out);
 // out[j][v] = Dt[j][iperm[v]] - phi[src_pos[j]][j]: fields in caller vertex order, 0 at the source.
 void launch_gather_rows(hipStream_t st, int n, int ldk, int k, const double* Dt, const double* phi, const int* src_pos,
                         const int* iperm, double* out);

@@ -112,7 +112,7 @@ __global__ void delta_rhs_kernel(double* __restrict__ X, int ldk, const int* __r
 // Level-scheduled supernodal sweeps with partitioned-inverse panels
 // ---------------------------------------------------------------------------
 
-// Zero the forward-sweep rows (Y rows and update rows) of the listed supernodes.
+// Zero the forward-sweep rows (Y rows, and with U set the update rows) of the listed supernodes.
 __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__ nodes, int ldk,
                                                          const int* __restrict__ node_start,
                                                          const int* __restrict__ node_s, const int* __restrict__ node_b,
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
     const int rg = threadIdx.x >> 6;
     const int s = node_s[id], b = node_b[id], start = node_start[id];
     for (int r = rg; r < s; r += 4) Y[(int64_t)(start + r) * ldk + col] = 0.0;
+    if (!U) return;
     double* Ut = U + node_uoff[id] * ldk;
     for (int r = rg; r < b; r += 4) Ut[(int64_t)r * ldk + col] = 0.0;
 }
@@ -131,24 +132,29 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // Fused gradient -> normalise -> divergence (fastcw_lean.py:416-442), gathered per vertex
 // ---------------------------------------------------------------------------
 //
-// A workgroup owns kDivRows consecutive vertex rows and every column of the RHS block. It first
-// stages, cooperatively and in one pass, every incidence (face f, corner a) of its rows in LDS:
-// the byte offsets of the face's three vertex rows and five coefficients from the Gram matrix of
-// the face's gradient basis (g1 = grad B1, g2 = grad B2, ga = grad Ba) and its area A (extrinsic
-// or intrinsic geometry alike):
+// Per incidence (face f, corner a) of a vertex row the term needs five coefficients of the Gram
+// matrix of the face's gradient basis (g1 = grad B1, g2 = grad B2, ga = grad Ba) and its area A
+// (extrinsic or intrinsic geometry alike):
 //     |grad u|^2        = du1^2 |g1|^2 + du1 du2 (2 g1.g2) + du2^2 |g2|^2
 //     A (grad u . ga)   = du1 (A g1.ga) + du2 (A g2.ga)
 // with du1 = u1 - u0, du2 = u2 - u0, so the per-column work is two differences, two short dot
-// products and one reciprocal square root. The work is then (row, 64-column slice) pairs dealt
-// to the waves round robin (with 256 columns, wave w takes slice w of every row), one lane per
-// column: a row's field values are loaded kDivC incidences at a time with all loads in flight
-// together (32-bit offsets through a buffer resource, no address arithmetic), and every face
-// quantity comes from LDS (a broadcast read). Each row sums its incidences in CSR order, so the
-// result does not depend on scheduling.
+// products and one reciprocal square root. The coefficients are computed once per mesh
+// (incidence_geometry_kernel), in the incidence (CSR) order of the rows.
 //
-// Workgroups are handed to the 8 XCDs round robin; the grid maps them so that each XCD walks
-// its own contiguous run of row groups, and the neighbour rows that consecutive groups share
-// are read from that XCD's L2.
+// The rows are cut, once per mesh, into groups of consecutive rows whose incidences touch at most
+// kDivVerts distinct vertices (at most kDivInc incidences, kDivGroupRows rows); with the
+// nested-dissection order a group of about 20 rows of a degree-6 mesh touches about 57 vertices.
+// A workgroup takes one (group, 64-column slice): it loads each distinct vertex's 64 values once
+// into LDS (one 512-byte run per vertex, all loads in flight together) with the group's
+// coefficients and each incidence's three local vertex indices, then each wave sums rows of the
+// group, one lane per column, reading everything from LDS. A vertex row is read from L2 about 2.7
+// times per slice instead of 18 times (three per incidence). Each row sums its incidences in CSR
+// order, so the result does not depend on scheduling. A row that alone exceeds the limits (a
+// vertex of degree above 63) forms a group of its own that reads its values from global memory.
+//
+// Workgroups are handed to the 8 XCDs round robin; the grid maps them so that each XCD walks its
+// own contiguous run of (group, slice) items, the slices of a group one after another, and the
+// neighbour rows that consecutive groups share are read from that XCD's L2.
 // 1/sqrt(q) for q > 0: the hardware estimate refined by two Newton steps (relative error a few
 // units in the last place), with q scaled out of the range where the estimate loses accuracy.
 // It replaces a correctly rounded square root followed by a division.
@@ -161,104 +167,115 @@ __device__ inline double rsqrt_refined(double q) {
     return tiny ? y * 0x1p+128 : y;
 }
 
-constexpr int kDivRows = 16;   // rows per workgroup
-constexpr int kDivInc = 256;   // incidences staged per pass (a pass per 256 when rows are long)
-constexpr int kDivC = 6;       // incidences in flight per row step (interior vertices have about 6)
+// One incidence's term -A (grad u . ga) / |grad u| for one column (0 where grad u vanishes:
+// |grad u| > 1e-300, fastcw_lean.py:433, holds for every positive q). Branch-free, so that the
+// terms of several incidences interleave.
+__device__ inline double divergence_term(double u0, double u1, double u2, const double* G) {
+    const double du1 = u1 - u0;
+    const double du2 = u2 - u0;
+    const double num = fma(du2, G[1], du1 * G[0]);
+    const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
+    const double r = rsqrt_refined(qq > 0.0 ? qq : 1.0);
+    return qq > 0.0 ? -num * r : 0.0;
+}
+
 constexpr int kXcds = 8;
-constexpr unsigned kBufWord3 = 0x00020000u;  // gfx9 buffer descriptor word 3 (raw 32-bit loads)
+constexpr int kDivSlice = 64;  // columns per workgroup (one per lane)
+constexpr int kDivC = 6;       // incidences per row step (interior vertices have about 6)
 
-// BUF: the field block is below 2 GiB, so loads go through a buffer resource with 32-bit
-// offsets; otherwise through 64-bit pointers (offsets still below 4 GiB).
-template <bool BUF>
-__global__ __launch_bounds__(256) void divergence_kernel(int n, int ldk, int ngroups, int run,
-                                                         const int* __restrict__ vf_ptr, const int* __restrict__ vf,
-                                                         const int* __restrict__ faces, const double* __restrict__ area,
-                                                         const double* __restrict__ gram, const double* __restrict__ u,
-                                                         double* __restrict__ out) {
-    __shared__ uint4 rec[kDivInc];       // byte offsets of the face's vertex rows 0, 1, 2
-    __shared__ double geo[kDivInc][5];   // A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2
-    __shared__ int rptr[kDivRows + 1];
+// geo[5 e + 0..4] = A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2 of incidence e = vf[e] (face*3 + a).
+__global__ void incidence_geometry_kernel(int ninc, const int* __restrict__ vf, const double* __restrict__ area,
+                                          const double* __restrict__ gram, double* __restrict__ geo) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ninc) return;
+    const int code = vf[e];
+    const int f = code / 3, a = code - 3 * f;
+    const double* Gp = gram + 6 * (int64_t)f;  // G00 G01 G02 G11 G12 G22
+    const double A = area[f];
+    const double g1a = a == 0 ? Gp[1] : (a == 1 ? Gp[3] : Gp[4]);
+    const double g2a = a == 0 ? Gp[2] : (a == 1 ? Gp[4] : Gp[5]);
+    double* o = geo + 5 * (int64_t)e;
+    o[0] = A * g1a;
+    o[1] = A * g2a;
+    o[2] = Gp[3];
+    o[3] = 2.0 * Gp[4];
+    o[4] = Gp[5];
+}
 
-    const int ncs = ldk / 64;
-    const int grp = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
-    if (grp >= ngroups) return;
+__global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, int run, DivDev d, const double* __restrict__ u,
+                                                         double* __restrict__ out, int ldk) {
+    __shared__ double vals[kDivVerts][kDivSlice];  // the group's distinct vertex rows, this slice
+    __shared__ double sgeo[kDivInc][5];
+    __shared__ unsigned sloc[kDivInc];
+    __shared__ int rptr[kDivGroupRows + 1];
+
+    const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
+    if (item >= nitems) return;
+    const int g = item / ncs, cs = item - g * ncs;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int r0 = grp * kDivRows, r1 = min(n, r0 + kDivRows);
-    if (tid <= r1 - r0) rptr[tid] = vf_ptr[r0 + tid];
-    __syncthreads();
-    const int eb = rptr[0], ee = rptr[r1 - r0];
-    const int npairs = (r1 - r0) * ncs;
-    const unsigned row_bytes = (unsigned)ldk * 8u;
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void*)u, (short)0, BUF ? (int)(row_bytes * (unsigned)n) : 0, kBufWord3);
+    const int4 G = d.grp[g];
+    const int r0 = G.x, e0 = G.y, v0 = G.z, nv = G.w;
+    const int nr = d.grp[g + 1].x - r0;
+    const int col = cs * kDivSlice + lane;
 
-    for (int base = eb; base < ee; base += kDivInc) {
-        const int cnt = min(kDivInc, ee - base);
-        if (base != eb) __syncthreads();  // the previous pass is done with the staged data
-        if (tid < cnt) {
-            const int code = vf[base + tid];
-            const int f = code / 3, a = code - 3 * f;
-            const int i0 = faces[3 * (int64_t)f], i1 = faces[3 * (int64_t)f + 1], i2 = faces[3 * (int64_t)f + 2];
-            const double* Gp = gram + 6 * (int64_t)f;  // G00 G01 G02 G11 G12 G22
-            double G[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) G[c] = Gp[c];
-            const double A = area[f];
-            const double g1a = a == 0 ? G[1] : (a == 1 ? G[3] : G[4]);
-            const double g2a = a == 0 ? G[2] : (a == 1 ? G[4] : G[5]);
-            rec[tid] = make_uint4((unsigned)i0 * row_bytes, (unsigned)i1 * row_bytes, (unsigned)i2 * row_bytes, 0u);
-            geo[tid][0] = A * g1a;
-            geo[tid][1] = A * g2a;
-            geo[tid][2] = G[3];
-            geo[tid][3] = 2.0 * G[4];
-            geo[tid][4] = G[5];
+    if (nv < 0) {
+        // a row beyond the limits, alone: its values straight from global memory
+        if (w != 0) return;
+        const int e1 = d.vf_ptr[r0 + 1];
+        double acc = 0.0;
+        for (int e = e0; e < e1; ++e) {
+            const int f = d.vf[e] / 3;
+            const int* fv = d.faces + 3 * (int64_t)f;
+            const double u0 = u[(int64_t)fv[0] * ldk + col];
+            const double u1 = u[(int64_t)fv[1] * ldk + col];
+            const double u2 = u[(int64_t)fv[2] * ldk + col];
+            acc += divergence_term(u0, u1, u2, d.geo + 5 * (int64_t)e);
         }
-        __syncthreads();
-        for (int pr = w; pr < npairs; pr += 4) {
-            const int r = pr / ncs, cs = pr - r * ncs;
-            const unsigned cb = (unsigned)(cs * 64 + lane) * 8u;  // this lane's column, in bytes
-            const int rs = max(rptr[r], base), re = min(rptr[r + 1], base + cnt);
-            if (rs >= re) continue;  // uniform over the wave
-            // a row split over staging passes continues the running sum the lane stored before
-            double* o = out + (int64_t)(r0 + r) * ldk + cs * 64 + lane;
-            double acc = rs == rptr[r] ? 0.0 : *o;
-            for (int e = rs; e < re; e += kDivC) {
-                double u0[kDivC], u1[kDivC], u2[kDivC];
+        out[(int64_t)r0 * ldk + col] = acc;
+        return;
+    }
+
+    if (tid <= nr) rptr[tid] = d.vf_ptr[r0 + tid] - e0;
+    if (nv > 0) {
+        // wave w loads vertices w, w + 4, ...: all its loads first (in-range repeats of the last
+        // vertex pad the count), then the LDS stores
+        constexpr int Q = kDivVerts / 4;
+        double x[Q];
+        const int* gv = d.verts + v0;
 #pragma unroll
-                for (int q = 0; q < kDivC; ++q) {
-                    const uint4 R = rec[min(e + q, re - 1) - base];
-                    if (BUF) {
-                        u0[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.x + cb, 0, 0));
-                        u1[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.y + cb, 0, 0));
-                        u2[q] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, R.z + cb, 0, 0));
-                    } else {
-                        const char* ub = reinterpret_cast<const char*>(u) + cb;
-                        u0[q] = *reinterpret_cast<const double*>(ub + R.x);
-                        u1[q] = *reinterpret_cast<const double*>(ub + R.y);
-                        u2[q] = *reinterpret_cast<const double*>(ub + R.z);
-                    }
-                }
-                __builtin_amdgcn_sched_barrier(0);  // keep every load of the step ahead of the arithmetic
+        for (int q = 0; q < Q; ++q) x[q] = u[(int64_t)gv[min(w + 4 * q, nv - 1)] * ldk + col];
 #pragma unroll
-                for (int q = 0; q < kDivC; ++q) {
-                    // every step computes (a skip would pull its loads down to it), padding masked
-                    const double* G = geo[min(e + q, re - 1) - base];
-                    const double du1 = u1[q] - u0[q];
-                    const double du2 = u2[q] - u0[q];
-                    const double num = fma(du2, G[1], du1 * G[0]);
-                    const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
-                    // |grad u| > 1e-300 (fastcw_lean.py:433) holds for every positive qq
-                    const double term = -num * rsqrt_refined(qq);
-                    acc += (e + q < re && qq > 0.0) ? term : 0.0;
-                }
-            }
-            *o = acc;
+        for (int q = 0; q < Q; ++q)
+            if (w + 4 * q < nv) vals[w + 4 * q][lane] = x[q];
+    }
+    {
+        const int ni = d.vf_ptr[r0 + nr] - e0;
+        for (int i = tid; i < ni; i += blockDim.x) {
+            sloc[i] = d.loc[e0 + i];
+            const double* gp = d.geo + 5 * (int64_t)(e0 + i);
+#pragma unroll
+            for (int c = 0; c < 5; ++c) sgeo[i][c] = gp[c];
         }
     }
-    // rows without incidences (isolated vertices) still get their zero
-    for (int pr = w; pr < npairs; pr += 4) {
-        const int r = pr / ncs, col = (pr - r * ncs) * 64 + lane;
-        if (rptr[r] == rptr[r + 1]) out[(int64_t)(r0 + r) * ldk + col] = 0.0;
+    __syncthreads();
+    for (int r = w; r < nr; r += 4) {
+        const int rs = rptr[r], re = rptr[r + 1];
+        double acc = 0.0;
+        // kDivC incidences per step, all computed (the padding repeats the last one), then added
+        // in CSR order with the padding masked
+        for (int e = rs; e < re; e += kDivC) {
+            double t[kDivC];
+#pragma unroll
+            for (int q = 0; q < kDivC; ++q) {
+                const int ec = min(e + q, re - 1);
+                const unsigned L = sloc[ec];
+                t[q] = divergence_term(vals[L & 63u][lane], vals[(L >> 6) & 63u][lane], vals[(L >> 12) & 63u][lane],
+                                       sgeo[ec]);
+            }
+#pragma unroll
+            for (int q = 0; q < kDivC; ++q) acc += (e + q < re) ? t[q] : 0.0;
+        }
+        out[(int64_t)(r0 + r) * ldk + col] = acc;
     }
 }
 
@@ -340,6 +357,10 @@ void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t
     calibration_copy_kernel<<<dim3(4096), 256, 0, st>>>((const double4*)src, (double4*)dst, count);
 }
 
+void launch_set_delta(hipStream_t st, double* X, int ldk, const int* src_pos, int k) {
+    delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
+}
+
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k) {
     launch_fill(st, X, (int64_t)n * ldk, 0.0);
     delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
@@ -351,16 +372,18 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
     clear_rows_kernel<<<dim3(count, ldk / 64), 256, 0, st>>>(nodes, ldk, node_start, node_s, node_b, node_uoff, Y, U);
 }
 
-void launch_divergence(hipStream_t st, const MeshDev& mesh, int ldk, const double* u, double* out) {
-    const int ngroups = (mesh.n + kDivRows - 1) / kDivRows;
-    const int run = (ngroups + kXcds - 1) / kXcds;  // row groups per XCD
-    const uint64_t bytes = (uint64_t)mesh.n * (uint64_t)ldk * 8u;  // below 4 GiB (checked at creation)
-    if (bytes < (1ull << 31))
-        divergence_kernel<true><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.op_vf_ptr, mesh.op_vf,
-                                                                   mesh.op_faces, mesh.op_area, mesh.op_gram, u, out);
-    else
-        divergence_kernel<false><<<dim3(run * kXcds), 256, 0, st>>>(mesh.n, ldk, ngroups, run, mesh.op_vf_ptr, mesh.op_vf,
-                                                                    mesh.op_faces, mesh.op_area, mesh.op_gram, u, out);
+void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const double* area, const double* gram,
+                               double* geo) {
+    if (ninc <= 0) return;
+    incidence_geometry_kernel<<<grid1(ninc, 256), 256, 0, st>>>(ninc, vf, area, gram, geo);
+}
+
+void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out) {
+    if (d.ngroups <= 0) return;
+    const int ncs = ldk / kDivSlice;
+    const int nitems = d.ngroups * ncs;
+    const int run = (nitems + kXcds - 1) / kXcds;  // items per XCD
+    divergence_kernel<<<dim3(run * kXcds), 256, 0, st>>>(nitems, ncs, run, d, u, out, ldk);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,

@@ -153,9 +153,10 @@ class HeatContext:
         _lib.check(_lib.lib().fcw_set_profiling(self._h, 1 if on else 0))
 
     def stage_times(self):
-        out = np.zeros(5)
-        _lib.lib().fcw_stage_times(self._h, _lib.dptr(out), 5)
-        return dict(heat_solve=out[0], fused_div=out[1], poisson_solve=out[2], metrics=out[3], wall=out[4])
+        out = np.zeros(6)
+        _lib.lib().fcw_stage_times(self._h, _lib.dptr(out), 6)
+        return dict(heat_solve=out[0], fused_div=out[1], poisson_solve=out[2], metrics=out[3], wall=out[4],
+                    issue=out[5])
 
     def kernel_times(self):
         out = np.zeros(6)

@@ -123,8 +123,9 @@ int fcw_geometry_fields(fcw_ctx* ctx, const double* D, const int* src, int k, in
                         fcw_metrics_out* out);
 
 /* Accumulated seconds: [0] heat_solve, [1] fused_div, [2] poisson_solve, [3] metrics, [4] wall of the
- * last fcw_wiring_sources call (the stage keys of fastcw_lean.py:527-530 plus the metric stage).
- * Stage times are filled only when profiling is on. Returns the number of values written. */
+ * last fcw_wiring_sources call, [5] host time that call spent issuing its blocks (the stage keys of
+ * fastcw_lean.py:527-530 plus the metric stage). Stages 0-3 are filled only when profiling is on.
+ * Returns the number of values written. */
 int fcw_stage_times(fcw_ctx* ctx, double* out, int len);
 
 /* Per-kernel timing with HIP events around every launch on the context's stream (0 = off).
