@@ -179,6 +179,15 @@ __device__ inline double divergence_term(double u0, double u1, double u2, const 
     return qq > 0.0 ? -num * r : 0.0;
 }
 
+// 1/sqrt(q) for q well inside the range where the hardware estimate is accurate (q >= 2^-767):
+// the estimate refined by two Newton steps.
+__device__ inline double rsqrt_newton(double q) {
+    double y = __builtin_amdgcn_rsq(q);
+    y = fma(0.5 * y, fma(-q * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-q * y, y, 1.0), y);
+    return y;
+}
+
 constexpr int kXcds = 8;
 constexpr int kDivSlice = 64;  // columns per workgroup (one per lane)
 constexpr int kDivC = 6;       // incidences per row step (interior vertices have about 6)
@@ -205,9 +214,10 @@ __global__ void incidence_geometry_kernel(int ninc, const int* __restrict__ vf, 
 __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, int run, DivDev d, const double* __restrict__ u,
                                                          double* __restrict__ out, int ldk) {
     __shared__ double vals[kDivVerts][kDivSlice];  // the group's distinct vertex rows, this slice
-    __shared__ double sgeo[kDivInc][5];
+    __shared__ double sgeo[5][kDivInc];
     __shared__ unsigned sloc[kDivInc];
     __shared__ int rptr[kDivGroupRows + 1];
+    __shared__ int rloc[kDivGroupRows];  // local index of each row's own vertex
 
     const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
     if (item >= nitems) return;
@@ -236,6 +246,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     }
 
     if (tid <= nr) rptr[tid] = d.vf_ptr[r0 + tid] - e0;
+    if (tid < nr) rloc[tid] = d.rloc[r0 + tid];
     if (nv > 0) {
         // wave w loads vertices w, w + 4, ...: all its loads first (in-range repeats of the last
         // vertex pad the count), then the LDS stores
@@ -254,13 +265,22 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             sloc[i] = d.loc[e0 + i];
             const double* gp = d.geo + 5 * (int64_t)(e0 + i);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) sgeo[i][c] = gp[c];
+            for (int c = 0; c < 5; ++c) sgeo[c][i] = gp[c];
         }
     }
     __syncthreads();
     for (int r = w; r < nr; r += 4) {
         const int rs = rptr[r], re = rptr[r + 1];
+        // Each term is unchanged when the values are scaled by a power of two (exactly: the
+        // differences, the quadratic form and the reciprocal square root scale exactly), so the
+        // row's values are scaled by its own vertex's magnitude: then |grad u|^2 stays far above
+        // the range where the hardware estimate needs the rescaling of rsqrt_refined. A wave that
+        // meets such a value anyway (a vanishing gradient) sums the row again the general way.
+        int ex = 0;
+        (void)frexp(vals[rloc[r]][lane], &ex);
+        const double sc = ldexp(1.0, -ex);
         double acc = 0.0;
+        bool tiny = false;
         // kDivC incidences per step, all computed (the padding repeats the last one), then added
         // in CSR order with the padding masked
         for (int e = rs; e < re; e += kDivC) {
@@ -269,11 +289,27 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             for (int q = 0; q < kDivC; ++q) {
                 const int ec = min(e + q, re - 1);
                 const unsigned L = sloc[ec];
-                t[q] = divergence_term(vals[L & 63u][lane], vals[(L >> 6) & 63u][lane], vals[(L >> 12) & 63u][lane],
-                                       sgeo[ec]);
+                const double u0 = vals[L & 63u][lane];
+                const double du1 = (vals[(L >> 6) & 63u][lane] - u0) * sc;
+                const double du2 = (vals[(L >> 12) & 63u][lane] - u0) * sc;
+                const double num = fma(du2, sgeo[1][ec], du1 * sgeo[0][ec]);
+                const double qq = fma(du1, fma(du2, sgeo[3][ec], du1 * sgeo[2][ec]), du2 * (du2 * sgeo[4][ec]));
+                tiny |= qq > 0.0 && qq < 0x1p-767;
+                // 0 where q <= 0 (the estimate is then NaN or infinite) by a bit mask, so that the
+                // compiler keeps the six terms branch-free
+                const uint64_t keep = qq > 0.0 ? ~0ull : 0ull;
+                t[q] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, -num * rsqrt_newton(qq)) & keep);
             }
 #pragma unroll
             for (int q = 0; q < kDivC; ++q) acc += (e + q < re) ? t[q] : 0.0;
+        }
+        if (__any(tiny)) {
+            acc = 0.0;
+            for (int e = rs; e < re; ++e) {
+                const unsigned L = sloc[e];
+                const double G[5] = {sgeo[0][e], sgeo[1][e], sgeo[2][e], sgeo[3][e], sgeo[4][e]};
+                acc += divergence_term(vals[L & 63u][lane], vals[(L >> 6) & 63u][lane], vals[(L >> 12) & 63u][lane], G);
+            }
         }
         out[(int64_t)(r0 + r) * ldk + col] = acc;
     }
