@@ -48,7 +48,8 @@ class Options(ctypes.Structure):
         ("streams", ctypes.c_int),
         ("sparse_rhs", ctypes.c_int),
         ("robust", ctypes.c_int),
-        ("reserved", ctypes.c_int * 3),
+        ("split_depth", ctypes.c_int),
+        ("reserved", ctypes.c_int * 2),
     ]
 
 
@@ -72,6 +73,7 @@ class Info(ctypes.Structure):
         ("intrinsic_flips", ctypes.c_int),
         ("intrinsic_skipped", ctypes.c_int),
         ("mollify_delta", ctypes.c_double),
+        ("split_levels", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -93,6 +93,7 @@ struct BlockPlan {
 struct Workspace {
     hipStream_t stream = nullptr;
     DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
+    DBuf<double> part;  // partial products of the split-depth sweep launches
     DBuf<double> cpart;              // chunk_stats records (k x nch x kChunkStats)
     DBuf<float> X32, Y32, U32, Z32;  // refined solve (FCW_FP32_IR): FP32 right-hand side and sweeps
     DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
@@ -128,6 +129,8 @@ struct fcw_ctx {
     DBuf<int64_t> node_boff, node_poff, node_uoff;
     DBuf<fcw::LevelTile> fwd_tiles, bwd_tiles;
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
+    // depth parts per level of the dense FP64 sweeps (1: not split)
+    std::vector<int> fwd_split, bwd_split;
     bool fwd_idx_lds = true, bwd_idx_lds = true;  // every node's sweep indices fit in LDS
     DBuf<double> panel_heat, panel_pois;
     DBuf<float> panel_pois32;  // FCW_FP32_IR: the Poisson panels rounded to FP32
@@ -272,14 +275,14 @@ struct fcw_ctx {
             EventPair* e = ev_begin(w, kFwdGemm);
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
                                  node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0,
-                                 (int)sym.urows);
+                                 (int)sym.urows, plan ? 1 : fwd_split[l], w.part.p);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
             EventPair* e = ev_begin(w, kBwdGemm);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p,
-                                 bwd_idx_lds, Dt, n);
+                                 bwd_idx_lds, Dt, n, bwd_split[l], w.part.p);
             ev_end(w, e);
         }
         if (plan)
@@ -650,6 +653,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->streams = 8;
     opt->robust = 0;
     opt->sparse_rhs = 1;
+    opt->split_depth = 0;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -866,6 +870,40 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         }
         C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
         C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
+        C->fwd_split.assign(S.nlevels, 1);
+        C->bwd_split.assign(S.nlevels, 1);
+        const bool split = opt.split_depth > 0 || (opt.split_depth == 0 && opt.streams <= 2);
+        if (split) {
+            // Split the depth of a level's launch when it has few workgroups and a long depth: aim
+            // at about kSplitTarget workgroups, at least two depth steps per part, at most 8 parts.
+            // With several blocks in flight the narrow levels of one block already run beside the
+            // wide launches of others, and the split's extra workgroups cost throughput, so it is
+            // on by default only for one or two streams.
+            constexpr int kSplitTarget = 512;
+            const int ncol = C->ldk / fcw::kTileN;
+            auto choose = [&](int ntiles, int kmax) {
+                const int wgs = (ntiles + 7) / 8 * 8 * ncol;
+                if (ntiles <= 0 || wgs >= kSplitTarget) return 1;
+                return std::max(1, std::min({8, kSplitTarget / wgs, kmax / (2 * fcw::kTileK)}));
+            };
+            size_t part_elems = 0;
+            for (int l = 0; l < S.nlevels; ++l) {
+                int kf = 0, kb = 0;
+                for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+                    const int id = S.level_nodes[i];
+                    kf = std::max(kf, S.node_s[id]);
+                    kb = std::max(kb, S.node_s[id] + S.node_b[id]);
+                }
+                const int nf = C->fwd_tile_off[l + 1] - C->fwd_tile_off[l];
+                const int nb = C->bwd_tile_off[l + 1] - C->bwd_tile_off[l];
+                if (l > 0) C->fwd_split[l] = choose(nf, kf);  // the leaf level never splits
+                C->bwd_split[l] = choose(nb, kb);
+                if (C->fwd_split[l] > 1) part_elems = std::max(part_elems, (size_t)nf * ncol * C->fwd_split[l] * 4096);
+                if (C->bwd_split[l] > 1) part_elems = std::max(part_elems, (size_t)nb * ncol * C->bwd_split[l] * 4096);
+            }
+            for (auto& w : C->ws) w->part.alloc(std::max<size_t>(1, part_elems), tb);
+        }
+        for (int l = 0; l < S.nlevels; ++l) C->info.split_levels += (C->fwd_split[l] > 1) + (C->bwd_split[l] > 1);
         const int max_s = S.nnodes ? *std::max_element(S.node_s.begin(), S.node_s.end()) : 0;
         C->fwd_idx_lds = max_s <= fcw::kSweepIdxCap;
         C->bwd_idx_lds = S.max_front <= 2 * fcw::kSweepIdxCap;

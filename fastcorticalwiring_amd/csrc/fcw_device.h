@@ -137,13 +137,16 @@ void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, con
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds, bool leaves, int zrow);
+                     double* U, bool idx_lds, bool leaves, int zrow, int nsplit = 1, double* part = nullptr);
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds,
-                     double* Dt, int ldd);
+                     double* Dt, int ldd, int nsplit = 1, double* part = nullptr);
 // Forward sweeps: U has a row zrow past the update rows that stays zero (it stands in for a
-// missing child contribution, so every gather is an unconditional load).
+// missing child contribution, so every gather is an unconditional load). nsplit > 1 (FP64, not
+// the leaf level): the depth is split over nsplit workgroups per (tile, slice) whose partial
+// products go to `part` (ntiles x ldk/64 x nsplit x 4096 doubles), then added in order by a second
+// launch that writes the results.
 // Backward sweeps with Dt set also write the FP64 solution source-major, Dt[col * ldd + position]
 // (unshifted; consumers subtract the source's own value).
 // FP32 forms of the sweeps (refined solve): FP32 panels and right-hand sides, FP32 matrix-core

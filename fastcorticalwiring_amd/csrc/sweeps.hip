@@ -67,17 +67,27 @@ enum class Sweep { Forward, Backward };
 // slices of a tile one after another: the slices read the same panel rows, and the row tiles of
 // one node the same right-hand-side rows, which the first of them brings into that XCD's L2 for
 // the others. Workgroups past the last tile exit.
+//
+// Split depth (SPLIT, FP64 only) for the narrow upper levels, whose few workgroups each walk a long
+// depth: SPLIT = 1 gives each (tile, slice) nsplit workgroups, part p multiplying depth steps
+// [p kc, (p + 1) kc) and storing its 64 x 64 partial product to `part`; SPLIT = 2 (the next launch,
+// one workgroup per (tile, slice)) adds the nsplit partials in part order, then runs the ordinary
+// epilogue. The order is fixed, so the result does not depend on scheduling; it differs from the
+// unsplit product only in rounding.
 constexpr int kXcds = 8;
 
-template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS, bool DT = false>
+template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS, bool DT = false, int SPLIT = 0>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     int ntiles, const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
     const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
-    T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd, int zrow) {
+    T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd, int zrow, int nsplit, double* __restrict__ part) {
+    static_assert(SPLIT == 0 || sizeof(T) == sizeof(double), "split depth is FP64 only");
     const int ncol = ldk / kTileN;
-    const int xcd = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int xcd = blockIdx.x % kXcds;
+    const int ps = SPLIT == 1 ? (int)(blockIdx.x / kXcds) % nsplit : 0;  // this workgroup's depth part
+    const int j = SPLIT == 1 ? (int)(blockIdx.x / kXcds) / nsplit : (int)(blockIdx.x / kXcds);
     const int run = (ntiles + kXcds - 1) / kXcds;  // tiles per XCD
     const int tile = xcd * run + j / ncol;
     if (tile >= ntiles) return;
@@ -90,6 +100,16 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     const int wm0 = (wave & 1) * 32, wn0 = (wave >> 1) * 32;
     const int M = (MODE == Sweep::Forward) ? f : s;   // output rows of this node
     const int K = (MODE == Sweep::Forward) ? s : f;   // depth
+    // depth range of this workgroup (all of it unless split)
+    int kbeg = 0, kend = K;
+    if (SPLIT == 1) {
+        const int kc = ((K + nsplit - 1) / nsplit + kTileK - 1) / kTileK * kTileK;
+        kbeg = min(K, ps * kc);
+        kend = min(K, kbeg + kc);
+    }
+    // 16 accumulator values per thread for each (tile, slice, part)
+    double* const pslot =
+        SPLIT ? part + (((int64_t)tile * ncol + (j - (j / ncol) * ncol)) * nsplit) * (kThreads * 16) + tid * 16 : nullptr;
 
     static_assert(kTileM == kTileN, "A and B stages share one LDS array");
     __shared__ T sbuf[2][kTileK][kTileM + kPad];  // A and B stages; the epilogue's transpose after the loop
@@ -98,7 +118,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     __shared__ int sidx[2 * kIdxCap];
     // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
     // contributor rows for each of the node's s own rows.
-    if (IDX_LDS) {
+    if (IDX_LDS && SPLIT != 2) {
         if (MODE == Sweep::Backward) {
             // source row of every depth index: own rows as positions, boundary rows as -(pos + 1).
             // The boundary-map loads are issued together from clamped indices (only the uniform
@@ -211,12 +231,21 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         for (int j = 0; j < 2; ++j) acc[i][j] = typename Mfma<T>::acc_t{T(0), T(0), T(0), T(0)};
 
     const int lrow = lane & 15, lk = lane >> 4;
-    if (K > 0) fetch(0);
-    for (int k0 = 0; k0 < K; k0 += kTileK) {
+    if (SPLIT == 2) {
+        // the partial products of the depth parts, added in part order
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q >> 3][(q >> 2) & 1][q & 3] = T(pslot[q]);
+        for (int p = 1; p < nsplit; ++p)
+#pragma unroll
+            for (int q = 0; q < 16; ++q) acc[q >> 3][(q >> 2) & 1][q & 3] += T(pslot[(int64_t)p * (kThreads * 16) + q]);
+        kend = 0;  // no depth steps of its own
+    }
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kTileK) {
         __syncthreads();
         stash();
         __syncthreads();
-        if (k0 + kTileK < K) fetch(k0 + kTileK);
+        if (k0 + kTileK < kend) fetch(k0 + kTileK);
 #pragma unroll
         for (int k4 = 0; k4 < kTileK; k4 += 4) {
             const T a0 = As[k4 + lk][wm0 + lrow];
@@ -228,6 +257,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
             acc[1][0] = Mfma<T>::step(a1, b0, acc[1][0]);
             acc[1][1] = Mfma<T>::step(a1, b1, acc[1][1]);
         }
+    }
+    if (SPLIT == 1) {
+        double* const slot = pslot + (int64_t)ps * (kThreads * 16);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) slot[q] = (double)acc[q >> 3][(q >> 2) & 1][q & 3];
+        return;
     }
 
     if (MODE == Sweep::Forward) {
@@ -345,40 +380,65 @@ template <typename T>
 void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
               const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
               const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves,
-              int zrow) {
+              int zrow, int nsplit, double* part) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));  // 8 runs of ceil(ntiles / 8)
-#define FCW_FWD(LDS, KIDS)                                                                                   \
-    sweep_kernel<T, Sweep::Forward, LDS, KIDS><<<grid, kThreads, 0, st>>>(                                  \
+    const dim3 grid_split(grid.x * (nsplit > 1 ? nsplit : 1));
+#define FCW_FWD(LDS, KIDS, SPL, G)                                                                            \
+    sweep_kernel<T, Sweep::Forward, LDS, KIDS, false, SPL><<<G, kThreads, 0, st>>>(                           \
         ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
-        panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0, zrow)
+        panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0, zrow, nsplit, part)
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (nsplit > 1 && !leaves) {
+            if (idx_lds) {
+                FCW_FWD(true, true, 1, grid_split);
+                FCW_FWD(true, true, 2, grid);
+            } else {
+                FCW_FWD(false, true, 1, grid_split);
+                FCW_FWD(false, true, 2, grid);
+            }
+            return;
+        }
+    }
     if (leaves)
-        FCW_FWD(true, false);
+        FCW_FWD(true, false, 0, grid);
     else if (idx_lds)
-        FCW_FWD(true, true);
+        FCW_FWD(true, true, 0, grid);
     else
-        FCW_FWD(false, true);
+        FCW_FWD(false, true, 0, grid);
 #undef FCW_FWD
 }
 
 template <typename T>
 void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
               const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const T* panel,
-              const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode, double* Dt, int ldd) {
+              const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode, double* Dt, int ldd, int nsplit,
+              double* part) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
-#define FCW_BWD(LDS, DTW)                                                                                       \
-    sweep_kernel<T, Sweep::Backward, LDS, true, DTW><<<grid, kThreads, 0, st>>>(                               \
+    const dim3 grid_split(grid.x * (nsplit > 1 ? nsplit : 1));
+#define FCW_BWD(LDS, DTW, SPL, G)                                                                               \
+    sweep_kernel<T, Sweep::Backward, LDS, true, DTW, SPL><<<G, kThreads, 0, st>>>(                             \
         ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, \
-        Z, nullptr, acc64, acc_mode, Dt, ldd, 0)
-    if (idx_lds && Dt)
-        FCW_BWD(true, true);
-    else if (idx_lds)
-        FCW_BWD(true, false);
-    else if (Dt)
-        FCW_BWD(false, true);
-    else
-        FCW_BWD(false, false);
+        Z, nullptr, acc64, acc_mode, Dt, ldd, 0, nsplit, part)
+#define FCW_BWD_ALL(SPL, G)          \
+    if (idx_lds && Dt)               \
+        FCW_BWD(true, true, SPL, G); \
+    else if (idx_lds)                \
+        FCW_BWD(true, false, SPL, G);\
+    else if (Dt)                     \
+        FCW_BWD(false, true, SPL, G);\
+    else                             \
+        FCW_BWD(false, false, SPL, G)
+    if constexpr (sizeof(T) == sizeof(double)) {
+        if (nsplit > 1) {
+            FCW_BWD_ALL(1, grid_split);
+            FCW_BWD_ALL(2, grid);
+            return;
+        }
+    }
+    FCW_BWD_ALL(0, grid);
+#undef FCW_BWD_ALL
 #undef FCW_BWD
 }
 
@@ -387,9 +447,9 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
-                     double* U, bool idx_lds, bool leaves, int zrow) {
+                     double* U, bool idx_lds, bool leaves, int zrow, int nsplit, double* part) {
     fwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
-                     contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow);
+                     contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow, nsplit, part);
 }
 
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -397,15 +457,15 @@ void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
                      const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,
                      float* U, bool idx_lds, bool leaves, int zrow) {
     fwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
-                    contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow);
+                    contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow, 1, nullptr);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds,
-                     double* Dt, int ldd) {
+                     double* Dt, int ldd, int nsplit, double* part) {
     bwd_gemm<double>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
-                     idx_lds, nullptr, 0, Dt, ldd);
+                     idx_lds, nullptr, 0, Dt, ldd, nsplit, part);
 }
 
 void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
@@ -413,7 +473,7 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
                      const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
                      double* acc64, int acc_mode, double* Dt, int ldd) {
     bwd_gemm<float>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
-                    idx_lds, acc64, acc_mode, Dt, ldd);
+                    idx_lds, acc64, acc_mode, Dt, ldd, 1, nullptr);
 }
 
 }  // namespace fcw

@@ -61,14 +61,16 @@ def _backend_mixin(base, potpourri=False):
           ir_steps (1, refinement steps of 'fp32_ir'), leaf_size (64), device (0), host_threads (0),
           streams (1: this adapter solves one block per miss, so more workspaces would sit idle),
           use_robust (False: operators on the surface's own triangulation, as ConsistentBatchHeatEngine;
-          True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode).
+          True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode),
+          split_depth (0: narrow tree levels split their depth when streams <= 2, 1 always, -1 never).
         Sources are solved in RHS blocks: a miss solves the next ``block`` sources of the planned
         order (default: the reference's BFS order, core_analysis.py:1384-1417), so the caller's
         one-source-at-a-time loop reads from the cache.
         """
 
         _ALLOWED = {"t_coef", "diffusion_length_mm", "poisson_shift", "block", "precision", "leaf_size", "device",
-                    "ir_steps", "host_threads", "streams", "use_robust"} | ({"allow_eigen_fallback"} if potpourri else set())
+                    "ir_steps", "host_threads", "streams", "use_robust", "split_depth"} | (
+                        {"allow_eigen_fallback"} if potpourri else set())
         _NAME = POTPOURRI_NAME if potpourri else ENGINE_NAME
 
         def __init__(self, vertices, faces, **engine_kwargs):
@@ -132,7 +134,8 @@ def _backend_mixin(base, potpourri=False):
                                     block=int(kw.pop("block", 256)), leaf_size=int(kw.pop("leaf_size", 64)),
                                     precision=precision, ir_steps=int(kw.pop("ir_steps", 1)),
                                     device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)),
-                                    streams=int(kw.pop("streams", 1)), robust=robust)
+                                    streams=int(kw.pop("streams", 1)), robust=robust,
+                                    split_depth=int(kw.pop("split_depth", 0)))
             self.t = self._ctx.t
             self._plan = None
             self._plan_pos = None

@@ -57,7 +57,10 @@ typedef struct {
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
     int robust;            /* 1: operators on the intrinsic Delaunay triangulation of the mollified surface,
                               potpourri3d's use_robust=True (distance_engines.py:78-79, 119-125); default 0 */
-    int reserved[3];
+    int split_depth;       /* FP64 sweeps of the narrow upper tree levels split their depth over several
+                              workgroups (lower latency per block, more work in all): 1 on, -1 off, 0 (default)
+                              on when streams <= 2 */
+    int reserved[2];
 } fcw_options;
 
 typedef struct {
@@ -73,6 +76,7 @@ typedef struct {
     int intrinsic_flips;        /* edge flips made (robust) */
     int intrinsic_skipped;      /* non-Delaunay edges left because flipping them would fold a face */
     double mollify_delta;       /* length added to every edge by the mollification (robust) */
+    int split_levels;           /* sweep launches per block whose depth is split (0: none) */
 } fcw_info;
 
 typedef struct {

@@ -110,6 +110,35 @@ def test_streams_and_sparse_rhs_give_identical_distances(c1):
         assert np.max(np.abs(D - ref)) <= 1e-12 * np.max(ref), (streams, sparse)
 
 
+def test_split_depth_sweeps_match_unsplit(c2):
+    """The narrow levels' depth split over several workgroups (partial products added in a fixed
+    order by a second launch) changes only rounding: distances agree with the unsplit sweeps to
+    1e-13 relative, are reproducible run to run and do not depend on the block's composition, and
+    the wiring metrics agree to 1e-9."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    s, mesh, ctx, H = c2
+    src = np.arange(0, mesh.n, 11)[:256]
+    with HeatContext(mesh.V, mesh.F, block=128, streams=1, split_depth=-1) as off, \
+            HeatContext(mesh.V, mesh.F, block=128, streams=1, split_depth=1) as on:
+        assert off.info["split_levels"] == 0 and on.info["split_levels"] > 0
+        print("split launches per block:", on.info["split_levels"], flush=True)
+        D0 = off.distances(src)
+        D1 = on.distances(src)
+        assert np.max(np.abs(D1 - D0)) <= 1e-13 * np.max(D0)
+        assert np.array_equal(on.distances(src), D1)
+        # the same sources in another block composition
+        D2 = on.distances(np.concatenate([src[64:128], src[:64]]))
+        assert np.array_equal(np.concatenate([D2[64:], D2[:64]]), D1[:128])
+        for c in (off, on):
+            _set_metrics(c, mesh, _scales())
+        w0, w1 = off.wiring(src), on.wiring(src)
+        ok = w0["scalars"][:, 0] == 0
+        assert np.array_equal(w0["scalars"][:, 0], w1["scalars"][:, 0]) and ok.sum() > 100
+        assert np.max(rel_err(w1["radii"][ok], w0["radii"][ok])) < 1e-9
+        assert np.max(rel_err(w1["perimeters"][ok], w0["perimeters"][ok])) < 1e-9
+
+
 def test_fp32_refined_poisson_meets_the_target(c2):
     """FCW_FP32_IR: Poisson system in FP32 sweeps refined against the FP64 residual; distances
     within 1e-5 relative of the FP64 oracle (d > 5h, as compare_fastcw_lean.py:235-239), and
