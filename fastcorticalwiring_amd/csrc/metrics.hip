@@ -459,6 +459,14 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const float* gch_lo = have_ch ? md.ch_lo + (int64_t)j * md.nch : nullptr;
     const float* gch_hi = have_ch ? md.ch_hi + (int64_t)j * md.nch : nullptr;
     const bool chunked = have_ch || m <= kMaxChunks * kWave * UNR;
+    // The face passes test every chunk's range before loading its faces, one chunk per wave step:
+    // from LDS that test costs a few cycles, from global memory a full load latency per step.
+    const bool ch_in_lds = !have_ch || md.nch <= kMaxChunks;
+    if (have_ch && ch_in_lds)
+        for (int c = tid; c < md.nch; c += NT) {
+            ch_lo[c] = gch_lo[c];
+            ch_hi[c] = gch_hi[c];
+        }
     // With chunk ranges and a finite field, only faces that start below `cut` are histogrammed:
     // the chunks entirely below bin K already hold the largest target's area, so every radius,
     // bracket, band and band capacity lies below bin K + 3 (eight bins of slack are kept).
@@ -502,7 +510,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
     const double cut = s_cut;
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
-        if (have_ch && (double)gch_lo[base / (kWave * UNR)] >= cut) continue;  // starts beyond every band
+        if (have_ch && (double)(ch_in_lds ? ch_lo : gch_lo)[base / (kWave * UNR)] >= cut) continue;  // beyond every band
         FaceBatch fb;
         load_faces<true>(faces, mesh.larea, d, sh, m, base, lane, fb);
         double cmin = INFINITY, cmax = -INFINITY;
@@ -721,8 +729,8 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
         if (chunked) {
             const int c = base / (kWave * UNR);
-            const double clo = have_ch ? (double)gch_lo[c] : (double)ch_lo[c];
-            const double chi = have_ch ? (double)gch_hi[c] : (double)ch_hi[c];
+            const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo[c];
+            const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi[c];
             if (chi < skip_lo || clo >= skip_hi) continue;  // meets no band
         }
         FaceBatch fb;
