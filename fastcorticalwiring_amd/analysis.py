@@ -72,7 +72,7 @@ class WiringAnalysis:
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
                                ir_steps=int(kw.pop("ir_steps", 1)), device=int(kw.pop("device", 0)),
                                host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)),
-                               robust=self.use_robust)
+                               robust=self.use_robust, split_depth=int(kw.pop("split_depth", 0)))
         if kw:
             raise ValueError(f"Unsupported engine kwargs: {', '.join(sorted(kw))}")
         self.field_flag = np.zeros(self.n_vertices_full, dtype=np.uint8)
@@ -91,6 +91,10 @@ class WiringAnalysis:
     normalize_scales = staticmethod(mp.normalize_scales)
     scale_token = staticmethod(mp.scale_token)
     boundary_area_loss_fraction = staticmethod(mp.boundary_area_loss_fraction)
+
+    def close(self):
+        """Free the device context (the result arrays stay); further computation raises."""
+        self.ctx.close()
 
     def compute_all_wiring_costs(self, scale=None, vertex_subset=None, n_samples_between_scales=3,
                                  boundary_cap_fraction=None, verbose=False):

@@ -150,9 +150,11 @@ def engine_kwargs_for(engine, kw):
 def run_single_surface(*, standard, surface_path, mask_path, output_dir, output_basename, output_label, subject_dir,
                        subject_id, hemi, surf_type, custom_label, no_mask, output_format, engine, engine_kwargs,
                        scale, eps, overwrite, n_samples_between_scales, boundary_cap_fraction, verbose_timing=False,
-                       sample=None, sample_kind="frac", sample_method=None, vertex_list=None):
+                       sample=None, sample_kind="frac", sample_method=None, vertex_list=None, admit=None):
     """One surface from loading to written files (fastcw.py:159-348); returns the paths written
-    ([] when outputs exist and overwrite is off)."""
+    ([] when outputs exist and overwrite is off). admit(n_cortical_vertices), when given, is called
+    before the device work starts and returns the function to call when it has ended (the batch
+    runner's admission)."""
     from .analysis import WiringAnalysis
 
     V, F, mask, meta = inputs.load_surface_and_mask(standard=standard, surface_path=surface_path, mask_path=mask_path,
@@ -204,9 +206,16 @@ def run_single_surface(*, standard, surface_path, mask_path, output_dir, output_
     elif sample_frac is not None or sample_count is not None:
         subset = inputs.select_vertex_subset(V, mask, sample_frac=sample_frac, sample_count=sample_count,
                                              sample_method=method)
-    an = WiringAnalysis(V, F, mask, engine_kwargs=engine_kwargs, eps=eps, metadata=meta)
-    an.compute_all_wiring_costs(scale=scales, vertex_subset=subset, n_samples_between_scales=n_samples_between_scales,
-                                boundary_cap_fraction=boundary_cap_fraction, verbose=verbose_timing)
+    release = admit(int(np.count_nonzero(mask))) if admit is not None else None
+    try:
+        an = WiringAnalysis(V, F, mask, engine_kwargs=engine_kwargs, eps=eps, metadata=meta)
+        an.compute_all_wiring_costs(scale=scales, vertex_subset=subset,
+                                    n_samples_between_scales=n_samples_between_scales,
+                                    boundary_cap_fraction=boundary_cap_fraction, verbose=verbose_timing)
+        an.close()  # device memory is given back before the admission slot
+    finally:
+        if release is not None:
+            release()
     return outputs.save_analysis_outputs(an, output_dir, kinds, csv_filename, stem)
 
 

@@ -24,7 +24,8 @@ import numpy as np
 from . import mesh_prep as mp
 from . import meshes
 
-__all__ = ["HemisphereRequest", "RequestResult", "ServingEngine", "estimate_device_bytes", "stream_requests"]
+__all__ = ["Admission", "HemisphereRequest", "RequestResult", "ServingEngine", "estimate_device_bytes",
+           "stream_requests"]
 
 
 @dataclass
@@ -66,6 +67,32 @@ def estimate_device_bytes(n, m, block=256, streams=2, scales=12, band_cap=6144, 
     mesh = m * 8 * 20 + n * 8 * 12
     outputs = n * (12 * 2 + 33 * 2 + 9 + 45) * 8
     return int(streams * per_ws + panels + mesh + outputs)
+
+
+class Admission:
+    """At most max_active hemispheres on the device at once, and a new one only while the sum of the
+    active ones' estimated device memory plus its own fits the budget (one is always admitted when
+    none is active, so an oversized request still runs, alone)."""
+
+    def __init__(self, max_active, hbm_budget_bytes):
+        self.max_active = int(max_active)
+        self.hbm_budget = int(hbm_budget_bytes)
+        self._lock = threading.Condition()
+        self.in_use = 0
+        self.active = 0
+
+    def admit(self, need):
+        with self._lock:
+            while self.active >= self.max_active or (self.active > 0 and self.in_use + need > self.hbm_budget):
+                self._lock.wait()
+            self.active += 1
+            self.in_use += need
+
+    def release(self, need):
+        with self._lock:
+            self.active -= 1
+            self.in_use -= need
+            self._lock.notify_all()
 
 
 class _Prepared:
@@ -110,23 +137,14 @@ class ServingEngine:
         self.streams = int(streams)
         self._prepare = prepare or (lambda req: _Prepared(req, self.block, self.streams, self.host_threads,
                                                            self.precision))
-        self._lock = threading.Condition()
-        self._in_use = 0
-        self._active = 0
+        self._admission = Admission(self.max_active, self.hbm_budget)
         self.trace = []
 
     def _admit(self, need):
-        with self._lock:
-            while self._active >= self.max_active or (self._active > 0 and self._in_use + need > self.hbm_budget):
-                self._lock.wait()
-            self._active += 1
-            self._in_use += need
+        self._admission.admit(need)
 
     def _release(self, need):
-        with self._lock:
-            self._active -= 1
-            self._in_use -= need
-            self._lock.notify_all()
+        self._admission.release(need)
 
     def _serve_one(self, req: HemisphereRequest, res: RequestResult, t0: float, prep_sem: threading.Semaphore):
         n_est = int(np.count_nonzero(req.cortex_mask))
