@@ -94,12 +94,32 @@ class MetricsOut(ctypes.Structure):
     ]
 
 
+class MeshPrepOut(ctypes.Structure):
+    _fields_ = [
+        ("n_sub", ctypes.c_int),
+        ("m_sub", ctypes.c_int),
+        ("n_boundary", ctypes.c_int),
+        ("n_adj", ctypes.c_int),
+        ("sub_to_orig", ctypes.POINTER(ctypes.c_int32)),
+        ("orig_to_sub", ctypes.POINTER(ctypes.c_int32)),
+        ("faces", ctypes.POINTER(ctypes.c_int32)),
+        ("vertices", ctypes.POINTER(ctypes.c_double)),
+        ("boundary", ctypes.POINTER(ctypes.c_int32)),
+        ("order", ctypes.POINTER(ctypes.c_int32)),
+        ("face_areas", ctypes.POINTER(ctypes.c_double)),
+        ("face_L", ctypes.POINTER(ctypes.c_double)),
+        ("vertex_areas", ctypes.POINTER(ctypes.c_double)),
+        ("adj_ptr", ctypes.POINTER(ctypes.c_int32)),
+        ("adj", ctypes.POINTER(ctypes.c_int32)),
+    ]
+
+
 # Every symbol include/fcw.h declares.
 EXPORTED_SYMBOLS = (
     "fcw_default_options", "fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block",
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
-    "fcw_calibration_copy", "fcw_host_intrinsic",
+    "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep",
 )
 
 _lib = None
@@ -140,10 +160,12 @@ def lib():
     L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     L.fcw_host_intrinsic.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp, _dp, _dp]
+    L.fcw_mesh_prep.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int,
+                                ctypes.c_int, ctypes.POINTER(MeshPrepOut)]
     for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
                  "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy",
-                 "fcw_host_intrinsic"):
+                 "fcw_host_intrinsic", "fcw_mesh_prep"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -213,3 +235,28 @@ def host_intrinsic(V, F, mollify=1e-6):
     check(lib().fcw_host_intrinsic(dptr(V), V.shape[0], iptr(F), m, float(mollify), iptr(faces), dptr(length),
                                    dptr(area), dptr(gram), dptr(stats)))
     return faces, length, area, gram, dict(flips=int(stats[0]), skipped=int(stats[1]), delta=float(stats[2]))
+
+
+def mesh_prep_device(V, F, mask, start=0, device=0):
+    """Mesh preparation on the device (fcw_mesh_prep): a dict with the cortical submesh
+    (vertices, faces, sub_to_orig, orig_to_sub), face_areas, face_L, vertex_areas, boundary, the
+    adjacency CSR (adj_ptr, adj) and the breadth-first order, each equal to mesh_prep's host result."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    M = np.ascontiguousarray(mask, dtype=np.uint8)
+    n, m = V.shape[0], F.shape[0]
+    buf = dict(sub_to_orig=np.empty(n, np.int32), orig_to_sub=np.empty(n, np.int32), faces=np.empty(3 * m, np.int32),
+               vertices=np.empty(3 * n), boundary=np.empty(n, np.int32), order=np.empty(n, np.int32),
+               face_areas=np.empty(m), face_L=np.empty(m), vertex_areas=np.empty(n),
+               adj_ptr=np.empty(n + 1, np.int32), adj=np.empty(max(1, 6 * m), np.int32))
+    out = MeshPrepOut()
+    for k, a in buf.items():
+        setattr(out, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double if a.dtype == np.float64 else ctypes.c_int32)))
+    check(lib().fcw_mesh_prep(dptr(V), n, iptr(F), m, M.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), int(start),
+                              int(device), ctypes.byref(out)))
+    ns, ms = out.n_sub, out.m_sub
+    return dict(vertices=buf["vertices"][:3 * ns].reshape(ns, 3), faces=buf["faces"][:3 * ms].reshape(ms, 3),
+                sub_to_orig=buf["sub_to_orig"][:ns], orig_to_sub=buf["orig_to_sub"], face_areas=buf["face_areas"][:ms],
+                face_L=buf["face_L"][:ms], vertex_areas=buf["vertex_areas"][:ns],
+                boundary=buf["boundary"][:out.n_boundary], order=buf["order"][:ns].astype(np.int64),
+                adj_ptr=buf["adj_ptr"][:ns + 1], adj=buf["adj"][:out.n_adj])

@@ -38,21 +38,25 @@ class WiringAnalysis:
         self.cortex_mask_full = M
         self.n_vertices_full = V.shape[0]
         t0 = time.perf_counter()
-        self.vertices, self.faces, self.sub_to_orig, self.orig_to_sub = mp.build_cortex_submesh(V, F, M)
+        # submesh, geometry, boundary, adjacency and source order on the device (fcw_mesh_prep),
+        # bit-identical to mesh_prep's host functions
+        d = _lib.mesh_prep_device(V, F, M, start=0, device=int(self.engine_kwargs.get("device", 0)))
+        self.vertices, self.faces, self.sub_to_orig, self.orig_to_sub = (d["vertices"], d["faces"], d["sub_to_orig"],
+                                                                          d["orig_to_sub"])
         self.n_vertices = self.vertices.shape[0]
         self.n_faces = self.faces.shape[0]
         if self.n_vertices == 0:
             raise ValueError("Cortex mask produced an empty cortical submesh.")
         if self.n_faces == 0:
             raise ValueError("No valid cortical faces remain after applying cortex mask.")
-        self.boundary_indices = mp.boundary_indices(self.faces)
-        self.face_areas, self.face_L, self.vertex_areas_sub = mp.face_geometry(self.vertices, self.faces)
+        self.boundary_indices = d["boundary"]
+        self.face_areas, self.face_L, self.vertex_areas_sub = d["face_areas"], d["face_L"], d["vertex_areas"]
         self.vertex_areas = np.zeros(self.n_vertices_full)
         self.vertex_areas[self.sub_to_orig] = self.vertex_areas_sub
         self._health_h = float(np.median(self.face_L))
-        self._lean_width = mp.default_bin_width(self.vertices, self.faces)
-        self._bfs_order = mp.bfs_order_all(self.n_vertices, self.faces)
-        self.mean_edge_length = mp.mean_edge_length(self.vertices, self.faces)
+        self._lean_width = mp.bin_width_from_lengths(self.face_L)
+        self._bfs_order = d["order"]
+        self.mean_edge_length = mp.mean_edge_from_adjacency(self.vertices, d["adj_ptr"], d["adj"])
         kw = dict(self.engine_kwargs)
         t = None
         self.diffusion_length_mm = None

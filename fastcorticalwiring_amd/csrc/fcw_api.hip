@@ -1149,6 +1149,28 @@ int fcw_calibration_copy(int64_t bytes, int device) {
     });
 }
 
+int fcw_mesh_prep(const double* V, int n, const int* F, int m, const unsigned char* mask, int start, int device,
+                  fcw_mesh_prep_out* out) {
+    return guard([&] {
+        if (!V || !F || !mask || !out) throw Fail("null argument");
+        if (n <= 0 || m <= 0) throw Fail("empty mesh");
+        if ((int64_t)m * 6 >= INT32_MAX) throw Fail("too many faces");
+        for (int64_t i = 0; i < 3LL * m; ++i)
+            if (F[i] < 0 || F[i] >= n) throw Fail("face index out of range");
+        if (!out->sub_to_orig || !out->orig_to_sub || !out->faces || !out->vertices || !out->boundary ||
+            !out->order || !out->face_areas || !out->face_L || !out->vertex_areas || !out->adj_ptr || !out->adj)
+            throw Fail("null output buffer");
+        fcw::MeshPrepOut o{0, 0, 0, 0, out->sub_to_orig, out->orig_to_sub, out->faces, out->vertices, out->boundary,
+                           out->order, out->face_areas, out->face_L, out->vertex_areas, out->adj_ptr, out->adj};
+        std::string err;
+        if (fcw::mesh_prep_device(n, V, m, F, mask, start, device, o, err)) throw Fail(err);
+        out->n_sub = o.n_sub;
+        out->m_sub = o.m_sub;
+        out->n_boundary = o.n_boundary;
+        out->n_adj = o.n_adj;
+    });
+}
+
 int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double mollify, int* faces_out, double* len_out,
                        double* area_out, double* gram_out, double* stats) {
     return guard([&] {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include <cstdint>
 
 namespace fcw {
@@ -82,6 +84,25 @@ struct DivDev {
     const int* vf;         // face*3 + corner
     const int* faces;      // operator faces, positions
 };
+
+// Mesh preparation on the device (prep.hip): caller-owned host buffers, sized for the input mesh
+// (n vertices, m faces, adjacency 6m).
+struct MeshPrepOut {
+    int n_sub, m_sub, n_boundary, n_adj;
+    int* sub_to_orig;     // n_sub
+    int* orig_to_sub;     // n (-1 outside the submesh)
+    int* faces;           // 3 m_sub, submesh numbering
+    double* vertices;     // 3 n_sub
+    int* boundary;        // n_boundary, ascending
+    int* order;           // n_sub, breadth-first order over every component
+    double* face_areas;   // m_sub
+    double* face_L;       // m_sub, longest edge
+    double* vertex_areas; // n_sub
+    int* adj_ptr;         // n_sub + 1
+    int* adj;             // n_adj, ascending per vertex
+};
+int mesh_prep_device(int n, const double* V, int m, const int* F, const unsigned char* mask, int start, int device,
+                     MeshPrepOut& out, std::string& err);
 
 struct MetricsDev {
     int S, q, nb;

@@ -157,6 +157,21 @@ def mean_edge_length(V, F):
     return float(np.mean(np.linalg.norm(V[E[:, 1]] - V[E[:, 0]], axis=1)))
 
 
+def mean_edge_from_adjacency(V, indptr, cols):
+    """mean_edge_length from a sorted adjacency CSR: the edges (a, b), a < b, in row order are the
+    sorted unique edges, so the mean is taken over the same values in the same order."""
+    indptr = np.asarray(indptr, dtype=np.int64)
+    rows = np.repeat(np.arange(indptr.size - 1), np.diff(indptr))
+    cols = np.asarray(cols, dtype=np.int64)
+    up = cols > rows
+    return float(np.mean(np.linalg.norm(V[cols[up]] - V[rows[up]], axis=1)))
+
+
+def bin_width_from_lengths(L, quantile=0.999, pad=1.05):
+    """default_bin_width from the faces' longest edges (face_geometry's L)."""
+    return float(np.quantile(L, quantile) * pad)
+
+
 def default_bin_width(V, F, quantile=0.999, pad=1.05):
     L = np.maximum.reduce([
         np.linalg.norm(V[F[:, 1]] - V[F[:, 0]], axis=1),

@@ -99,18 +99,21 @@ class _Prepared:
     def __init__(self, req: HemisphereRequest, block, streams, host_threads, precision):
         from .engine import HeatContext
 
+        from . import _lib
+
         V, F, M = mp.validate_mesh_inputs(req.vertices, req.faces, req.cortex_mask)
-        self.V, self.F, self.s2o, self.o2s = mp.build_cortex_submesh(V, F, M)
-        fa, fl, va = mp.face_geometry(self.V, self.F)
-        self.order = mp.bfs_order_all(self.V.shape[0], self.F)
+        d = _lib.mesh_prep_device(V, F, M)  # submesh, geometry, boundary and order on the device
+        self.V, self.F, self.s2o, self.o2s = d["vertices"], d["faces"], d["sub_to_orig"], d["orig_to_sub"]
+        fl, va = d["face_L"], d["vertex_areas"]
+        self.order = d["order"]
         self.ctx = HeatContext(self.V, self.F, block=block, precision=precision, host_threads=host_threads,
                                streams=streams)
         self.ctx_streams = streams
         targets = np.array(sorted(req.scales)) * float(va.sum())
         q = int(req.n_samples_between_scales)
         fracs = np.arange(1, q + 1) / (q + 1.0)
-        self.ctx.set_metrics(targets, fracs, mp.boundary_indices(self.F), va, health_h=float(np.median(fl)),
-                             bin_width=mp.default_bin_width(self.V, self.F),
+        self.ctx.set_metrics(targets, fracs, d["boundary"], va, health_h=float(np.median(fl)),
+                             bin_width=mp.bin_width_from_lengths(fl),
                              boundary_cap_fraction=req.boundary_cap_fraction)
 
 

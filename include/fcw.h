@@ -154,6 +154,33 @@ int fcw_calibration_copy(int64_t bytes, int device);
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale,
                           int leaf_size, const double* B, int k, double* X, double* stats);
 
+/* Mesh preparation on the device, the steps before the solves (core_analysis.py:651-683, 859-978,
+ * 1336-1417, which build_cortex_submesh, boundary, face geometry, adjacency and
+ * _compute_bfs_order_all do on the host): the cortical submesh (faces with all three vertices in
+ * `mask` and area above 1e-12, vertices in ascending original index), face areas (zero below the
+ * scale-aware threshold) and longest edges, vertex areas (each face's area / 3 per corner), the
+ * boundary vertices (on an edge of one face), the sorted adjacency and the breadth-first order from
+ * `start` (then from the smallest unvisited vertex), all bit-identical to the host computation.
+ * The caller owns every buffer, sized for the input: sub_to_orig, orig_to_sub, boundary, order,
+ * vertex_areas n; adj_ptr n + 1; vertices 3n; faces 3m; face_areas, face_L m; adj 6m. The counts
+ * are returned in the struct. */
+typedef struct {
+    int n_sub, m_sub, n_boundary, n_adj;
+    int* sub_to_orig;
+    int* orig_to_sub;
+    int* faces;
+    double* vertices;
+    int* boundary;
+    int* order;
+    double* face_areas;
+    double* face_L;
+    double* vertex_areas;
+    int* adj_ptr;
+    int* adj;
+} fcw_mesh_prep_out;
+int fcw_mesh_prep(const double* V, int n, const int* F, int m, const unsigned char* mask, int start, int device,
+                  fcw_mesh_prep_out* out);
+
 /* Host-only: the intrinsic Delaunay triangulation the robust operators are built on (the
  * triangulation inside potpourri3d's robust mode, distance_engines.py:119-125, which has no
  * callable counterpart). faces_out m x 3 (same vertices, flipped connectivity), len_out m x 3

@@ -105,6 +105,19 @@ def test_mesh_prep_matches_oracle(surf_c2):
     assert np.array_equal(a1, a2) and np.array_equal(w1, w2)
 
 
+def test_prep_helpers_from_adjacency_and_lengths(surf_c2):
+    """The analysis takes the mean edge and the bin width from the adjacency and the longest edges
+    that the device preparation returns: the same values as from the faces."""
+    from fastcorticalwiring_amd import mesh_prep as mp
+
+    V, F, M = mp.validate_mesh_inputs(surf_c2.vertices, surf_c2.faces, surf_c2.cortex_mask)
+    Vs, Fs, _, _ = mp.build_cortex_submesh(V, F, M)
+    ptr, cols = mp.adjacency_csr(Vs.shape[0], Fs)
+    _, L, _ = mp.face_geometry(Vs, Fs)
+    assert mp.mean_edge_from_adjacency(Vs, ptr, cols) == mp.mean_edge_length(Vs, Fs)
+    assert mp.bin_width_from_lengths(L) == mp.default_bin_width(Vs, Fs)
+
+
 def test_mesh_input_validation_errors():
     from fastcorticalwiring_amd import mesh_prep as mp
 
