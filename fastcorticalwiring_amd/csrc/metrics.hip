@@ -65,10 +65,6 @@ __device__ inline int wave_min_i(int v) {
     for (int o = kWave / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
     return v;
 }
-__device__ inline int wave_max_i(int v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-    return v;
-}
 
 __device__ inline uint64_t lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
@@ -756,30 +752,21 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             anyf |= ia[u] <= ib[u];
         }
         if (__ballot(anyf) == 0ull) continue;  // no band face in this wave's step
+        // Each lane takes a slot in each of its faces' bands with its own LDS atomic: the lanes'
+        // (few) bands are walked together, so the wave waits for one atomic per band a face holds
+        // rather than for one per band the whole step spans.
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            const bool mem = ia[u] <= ib[u];
-            if (__ballot(mem) == 0ull) continue;
-            const int imin = wave_min_i(mem ? ia[u] : NB);
-            const int imax = wave_max_i(mem ? ib[u] : -1);
-            for (int i = imin; i <= imax; ++i) {
-                const bool pred = mem && ia[u] <= i && i <= ib[u] && b_off[i] >= 0;
-                const uint64_t mask = __ballot(pred);
-                if (mask == 0ull) continue;
-                const int leader = __ffsll((long long)mask) - 1;
-                int slot0 = 0;
-                if (lane == leader) slot0 = atomicAdd(&b_cnt[i], __popcll(mask));
-                slot0 = __shfl(slot0, leader, kWave);
-                if (pred) {
-                    const int slot = slot0 + __popcll(mask & lanes_below(lane));
-                    if (slot < b_cap[i]) {
-                        double* bs = pool + b_off[i] + slot;
-                        bs[0] = sa_[u];
-                        bs[pstride] = sb_[u];
-                        bs[2 * pstride] = sc_[u];
-                        bs[3 * pstride] = fb.A[u];
-                        bs[4 * pstride] = (double)fb.f[u];
-                    }
+            for (int i = ia[u]; i <= ib[u]; ++i) {
+                if (b_off[i] < 0) continue;  // a band recomputed from every face
+                const int slot = atomicAdd(&b_cnt[i], 1);
+                if (slot < b_cap[i]) {
+                    double* bs = pool + b_off[i] + slot;
+                    bs[0] = sa_[u];
+                    bs[pstride] = sb_[u];
+                    bs[2 * pstride] = sc_[u];
+                    bs[3 * pstride] = fb.A[u];
+                    bs[4 * pstride] = (double)fb.f[u];
                 }
             }
         }
