@@ -141,6 +141,30 @@ __device__ inline void load_faces(const int* __restrict__ faces, const double* _
     __builtin_amdgcn_sched_barrier(0);  // every load of the step is issued before any use
 }
 
+// The chunks c < nchunks with keep(c) written in increasing order to list (block-wide, every thread
+// takes part); returns their number.
+template <typename Keep>
+__device__ int chunk_list(int nchunks, Keep keep, int* list, int* wcnt, int& nlist) {
+    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    if (tid == 0) nlist = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < nchunks; c0 += NT) {
+        const int c = c0 + tid;
+        const bool k = c < nchunks && keep(c);
+        const uint64_t mk = __ballot(k);
+        if (lane == 0) wcnt[wid] = __popcll(mk);
+        __syncthreads();
+        int before = nlist;
+        for (int w = 0; w < wid; ++w) before += wcnt[w];
+        if (k) list[before + __popcll(mk & lanes_below(lane))] = c;
+        __syncthreads();
+        if (tid == 0)
+            for (int w = 0; w < NW; ++w) nlist += wcnt[w];
+        __syncthreads();
+    }
+    return nlist;
+}
+
 struct Red {
     double d[NW][4];
     int i[NW][4];
@@ -316,6 +340,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __shared__ float ch_lo[kMaxChunks], ch_hi[kMaxChunks];  // per 256-face chunk: min dmin (rounded
                                                              // down), max dmax (rounded up)
     __shared__ int s_nbins, s_ovf;
+    __shared__ int s_list[kMaxChunks], s_wcnt[NW], s_nlist;  // face chunks a pass visits, in order
     __shared__ double s_skip_lo, s_skip_hi, s_cut;
     __shared__ double s_wb;
     __shared__ double s_scal[kSlotCount];
@@ -505,11 +530,10 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     }
     __syncthreads();
     const double cut = s_cut;
-    for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
-        if (have_ch && (double)(ch_in_lds ? ch_lo : gch_lo)[base / (kWave * UNR)] >= cut) continue;  // beyond every band
-        FaceBatch fb;
-        load_faces<true>(faces, mesh.larea, d, sh, m, base, lane, fb);
-        double cmin = INFINITY, cmax = -INFINITY;
+    // one wave step of the histogram pass (CONSEC face order); returns the step's distance range
+    auto hist_step = [&](const FaceBatch& fb, double& cmin, double& cmax) {
+        cmin = INFINITY;
+        cmax = -INFINITY;
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
             const double d0 = fb.d0[u], d1 = fb.d1[u], d2 = fb.d2[u];
@@ -525,16 +549,41 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                 cmax = fmax(cmax, hi);
             }
         }
-        if (!have_ch && chunked) {
-            // float bounds rounded outwards, reduced over the wave
-            float flo = __double2float_rd(cmin), fhi = __double2float_ru(cmax);
-            for (int o = kWave / 2; o > 0; o >>= 1) {
-                flo = fminf(flo, __shfl_xor(flo, o, kWave));
-                fhi = fmaxf(fhi, __shfl_xor(fhi, o, kWave));
-            }
-            if (lane == 0) {
-                ch_lo[base / (kWave * UNR)] = flo;
-                ch_hi[base / (kWave * UNR)] = fhi;
+    };
+    const int nchunks1 = (m + kWave * UNR - 1) / (kWave * UNR);
+    if (have_ch && ch_in_lds) {
+        // the chunks that start below the cut, listed in order; each wave loads its next chunk's
+        // faces while it adds up the current one
+        auto below_cut = [&](int c) { return (double)ch_lo[c] < cut; };
+        const int nlist = chunk_list(nchunks1, below_cut, s_list, s_wcnt, s_nlist);
+        FaceBatch cur, nxt;
+        int k = wid;
+        if (k < nlist) load_faces<true>(faces, mesh.larea, d, sh, m, s_list[k] * (kWave * UNR), lane, cur);
+        nxt = cur;
+        for (; k < nlist; k += NW) {
+            if (k + NW < nlist) load_faces<true>(faces, mesh.larea, d, sh, m, s_list[k + NW] * (kWave * UNR), lane, nxt);
+            double cmin, cmax;
+            hist_step(cur, cmin, cmax);
+            cur = nxt;
+        }
+    } else {
+        for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+            if (have_ch && (double)gch_lo[base / (kWave * UNR)] >= cut) continue;  // beyond every band
+            FaceBatch fb;
+            load_faces<true>(faces, mesh.larea, d, sh, m, base, lane, fb);
+            double cmin, cmax;
+            hist_step(fb, cmin, cmax);
+            if (!have_ch && chunked) {
+                // float bounds rounded outwards, reduced over the wave
+                float flo = __double2float_rd(cmin), fhi = __double2float_ru(cmax);
+                for (int o = kWave / 2; o > 0; o >>= 1) {
+                    flo = fminf(flo, __shfl_xor(flo, o, kWave));
+                    fhi = fmaxf(fhi, __shfl_xor(fhi, o, kWave));
+                }
+                if (lane == 0) {
+                    ch_lo[base / (kWave * UNR)] = flo;
+                    ch_hi[base / (kWave * UNR)] = fhi;
+                }
             }
         }
     }
@@ -722,15 +771,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
 
     // ---- phase 3: band faces, pre-sorted, contiguous per band -----------------------------
     const double skip_lo = s_skip_lo, skip_hi = s_skip_hi;
-    for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
-        if (chunked) {
-            const int c = base / (kWave * UNR);
-            const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo[c];
-            const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi[c];
-            if (chi < skip_lo || clo >= skip_hi) continue;  // meets no band
-        }
-        FaceBatch fb;
-        load_faces<false>(faces, mesh.larea, d, sh, m, base, lane, fb);
+    auto band_step = [&](const FaceBatch& fb) {
         int ia[UNR], ib[UNR];
         double sa_[UNR], sb_[UNR], sc_[UNR];
         bool anyf = false;
@@ -751,7 +792,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             sc_[u] = fc;
             anyf |= ia[u] <= ib[u];
         }
-        if (__ballot(anyf) == 0ull) continue;  // no band face in this wave's step
+        if (__ballot(anyf) == 0ull) return;  // no band face in this wave's step
         // Each lane takes a slot in each of its faces' bands with its own LDS atomic: the lanes'
         // (few) bands are walked together, so the wave waits for one atomic per band a face holds
         // rather than for one per band the whole step spans.
@@ -769,6 +810,33 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     bs[4 * pstride] = (double)fb.f[u];
                 }
             }
+        }
+    };
+    auto meets_bands = [&](int c) {
+        const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo[c];
+        const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi[c];
+        return !(chi < skip_lo || clo >= skip_hi);
+    };
+    const int nchunks = (m + kWave * UNR - 1) / (kWave * UNR);
+    if (chunked && nchunks <= kMaxChunks) {
+        // The chunks that meet the bands, listed in order; the waves take them in turn, each
+        // loading its next chunk's faces while it files the current one.
+        const int nlist = chunk_list(nchunks, meets_bands, s_list, s_wcnt, s_nlist);
+        FaceBatch cur, nxt;
+        int k = wid;
+        if (k < nlist) load_faces<false>(faces, mesh.larea, d, sh, m, s_list[k] * (kWave * UNR), lane, cur);
+        nxt = cur;
+        for (; k < nlist; k += NW) {
+            if (k + NW < nlist) load_faces<false>(faces, mesh.larea, d, sh, m, s_list[k + NW] * (kWave * UNR), lane, nxt);
+            band_step(cur);
+            cur = nxt;
+        }
+    } else {
+        for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+            if (chunked && !meets_bands(base / (kWave * UNR))) continue;
+            FaceBatch fb;
+            load_faces<false>(faces, mesh.larea, d, sh, m, base, lane, fb);
+            band_step(fb);
         }
     }
     __syncthreads();
