@@ -138,6 +138,12 @@ def lib():
         raise ImportError(
             f"libfcw.so not found at {LIB_PATH}; build it with "
             "`python -m fastcorticalwiring_amd.build` (hipcc, gfx950)")
+    # The host factorisation's OpenMP threads wait passively at their barriers: spinning waiters
+    # take the time slices of a team member that shares its core with other work (another
+    # context's factorisation, a CPU process), which made the factorisation ten times slower
+    # (L7: 1.5 s instead of 0.14 s with two busy cores). libgomp reads this when it is loaded,
+    # here with the library; a value the caller has set is kept.
+    os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
     L = ctypes.CDLL(LIB_PATH)
     L.fcw_default_options.argtypes = [ctypes.POINTER(Options)]
     L.fcw_default_options.restype = None

@@ -163,65 +163,124 @@ void trmm_right_lower(int b, int s, const double* Linv, int ldl, double* W, int 
 
 }  // namespace
 
+namespace {
+
+// One front of the multifrontal factorisation: assemble node id's columns and its children's
+// updates, factor, keep its own update matrix in upd[id] and store its panel. Returns 0, or 2 with
+// the first failing column of a front that is not positive definite in *bad_col.
+int factor_node(const Symbolic& S, int id, const double* lval, const double* mdiag, double l_scale, double m_scale,
+                double* panel, std::vector<std::vector<double>>& upd, std::vector<double>& front, bool par,
+                int* bad_col) {
+    const int s = S.node_s[id], b = S.node_b[id], f = s + b;
+    const int start = S.node_start[id], end = start + s;
+    const int* B = S.bidx.data() + S.node_boff[id];
+    front.assign((size_t)f * f, 0.0);
+    // original entries of the s columns (rows >= column)
+    for (int j = 0; j < s; ++j) {
+        const int col = start + j;
+        for (int e = S.rowptr[col]; e < S.rowptr[col + 1]; ++e) {
+            const int row = S.colind[e];
+            if (row < col) continue;
+            int r;
+            if (row < end) {
+                r = row - start;
+            } else {
+                const int* it = std::lower_bound(B, B + b, row);
+                if (it == B + b || *it != row) return 1;
+                r = s + (int)(it - B);
+            }
+            double v = l_scale * lval[e];
+            if (row == col) v += m_scale * mdiag[col];
+            front[r + (size_t)j * f] += v;
+        }
+    }
+    // extend-add children updates
+    for (int ci = S.child_ptr[id]; ci < S.child_ptr[id + 1]; ++ci) {
+        const int c = S.child_list[ci];
+        const int bc = S.node_b[c];
+        const int* cm = S.cmap.data() + S.node_boff[c];
+        const std::vector<double>& U = upd[c];
+        for (int jj = 0; jj < bc; ++jj) {
+            double* fc = front.data() + (size_t)cm[jj] * f;
+            const double* uc = U.data() + (size_t)jj * bc;
+            for (int ii = jj; ii < bc; ++ii) fc[cm[ii]] += uc[ii];
+        }
+        std::vector<double>().swap(upd[c]);
+    }
+    const int bad = partial_cholesky(f, s, front.data(), f, par);
+    if (bad >= 0) {
+        *bad_col = bad;
+        return 2;
+    }
+    if (b > 0) {
+        upd[id].resize((size_t)b * b);
+        for (int jj = 0; jj < b; ++jj)
+            std::memcpy(upd[id].data() + (size_t)jj * b, front.data() + s + (size_t)(s + jj) * f, sizeof(double) * b);
+    }
+    trtri_lower(s, front.data(), f, par);
+    trmm_right_lower(b, s, front.data(), f, front.data() + s, f, par);
+    double* P = panel + S.node_poff[id];
+    for (int j = 0; j < s; ++j) std::memcpy(P + (size_t)j * f, front.data() + (size_t)j * f, sizeof(double) * f);
+    return 0;
+}
+
+}  // namespace
+
+// The tree is factored level by level from the leaves (a node's children lie on lower levels).
+// A level with many fronts spreads them over the threads, one front per thread; a level with few
+// (the separators near the root, whose fronts are large) factors them one after another with the
+// threads inside each front's dense kernels. Every front's arithmetic is the same either way, so
+// the panels do not depend on the thread count.
 int factor(const Symbolic& S, const double* lval, const double* mdiag, double l_scale, double m_scale,
            double* panel, std::string& err, int nthreads) {
-    const int nn = S.nnodes;
-    std::vector<std::vector<double>> upd(nn);
+    std::vector<std::vector<double>> upd(S.nnodes);
     if (nthreads > 0) omp_set_num_threads(nthreads);
-    std::vector<double> front;
-    for (int id = 0; id < nn; ++id) {
-        const int s = S.node_s[id], b = S.node_b[id], f = s + b;
-        const int start = S.node_start[id], end = start + s;
-        const int* B = S.bidx.data() + S.node_boff[id];
-        front.assign((size_t)f * f, 0.0);
-        // original entries of the s columns (rows >= column)
-        for (int j = 0; j < s; ++j) {
-            const int col = start + j;
-            for (int e = S.rowptr[col]; e < S.rowptr[col + 1]; ++e) {
-                const int row = S.colind[e];
-                if (row < col) continue;
-                int r;
-                if (row < end) {
-                    r = row - start;
-                } else {
-                    const int* it = std::lower_bound(B, B + b, row);
-                    if (it == B + b || *it != row) { err = "pattern entry outside front"; return 1; }
-                    r = s + (int)(it - B);
+    const int nthr = omp_get_max_threads();
+    for (int L = 0; L < S.nlevels; ++L) {
+        const int lo = S.level_ptr[L], hi = S.level_ptr[L + 1];
+        // first failure of the level in node order (the same whichever thread meets it)
+        int fail_id = -1, fail_rc = 0, fail_col = -1;
+        auto report = [&](int id, int rc, int col) {
+            if (fail_id < 0 || id < fail_id) {
+                fail_id = id;
+                fail_rc = rc;
+                fail_col = col;
+            }
+        };
+        if (hi - lo >= 2 * nthr && nthr > 1) {
+#pragma omp parallel
+            {
+                std::vector<double> front;
+#pragma omp for schedule(dynamic, 1)
+                for (int i = lo; i < hi; ++i) {
+                    const int id = S.level_nodes[i];
+                    int col = -1;
+                    const int rc = factor_node(S, id, lval, mdiag, l_scale, m_scale, panel, upd, front, false, &col);
+                    if (rc) {
+#pragma omp critical(fcw_factor_fail)
+                        report(id, rc, col);
+                    }
                 }
-                double v = l_scale * lval[e];
-                if (row == col) v += m_scale * mdiag[col];
-                front[r + (size_t)j * f] += v;
+            }
+        } else {
+            std::vector<double> front;
+            for (int i = lo; i < hi; ++i) {
+                const int id = S.level_nodes[i];
+                const int s = S.node_s[id], f = s + S.node_b[id];
+                int col = -1;
+                const int rc = factor_node(S, id, lval, mdiag, l_scale, m_scale, panel, upd, front,
+                                           (double)f * f * s > 4e6, &col);
+                if (rc) report(id, rc, col);
             }
         }
-        // extend-add children updates
-        for (int ci = S.child_ptr[id]; ci < S.child_ptr[id + 1]; ++ci) {
-            const int c = S.child_list[ci];
-            const int bc = S.node_b[c];
-            const int* cm = S.cmap.data() + S.node_boff[c];
-            const std::vector<double>& U = upd[c];
-            for (int jj = 0; jj < bc; ++jj) {
-                double* fc = front.data() + (size_t)cm[jj] * f;
-                const double* uc = U.data() + (size_t)jj * bc;
-                for (int ii = jj; ii < bc; ++ii) fc[cm[ii]] += uc[ii];
-            }
-            std::vector<double>().swap(upd[c]);
+        if (fail_id >= 0) {
+            if (fail_rc == 1)
+                err = "pattern entry outside front";
+            else
+                err = "matrix not positive definite (node " + std::to_string(fail_id) + ", column " +
+                      std::to_string(fail_col) + ")";
+            return fail_rc;
         }
-        const bool par = (double)f * f * s > 4e6;
-        int bad = partial_cholesky(f, s, front.data(), f, par);
-        if (bad >= 0) {
-            err = "matrix not positive definite (node " + std::to_string(id) + ", column " + std::to_string(bad) + ")";
-            return 2;
-        }
-        if (b > 0) {
-            upd[id].resize((size_t)b * b);
-            for (int jj = 0; jj < b; ++jj)
-                std::memcpy(upd[id].data() + (size_t)jj * b, front.data() + s + (size_t)(s + jj) * f,
-                            sizeof(double) * b);
-        }
-        trtri_lower(s, front.data(), f, par);
-        trmm_right_lower(b, s, front.data(), f, front.data() + s, f, par);
-        double* P = panel + S.node_poff[id];
-        for (int j = 0; j < s; ++j) std::memcpy(P + (size_t)j * f, front.data() + (size_t)j * f, sizeof(double) * f);
     }
     return 0;
 }

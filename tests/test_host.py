@@ -219,3 +219,23 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_host_factorisation_independent_of_thread_count():
+    """The tree is factored level by level, wide levels one front per thread: the solution must be
+    bit-identical for any OpenMP thread count (each run in its own process)."""
+    script = (
+        "import hashlib, numpy as np\n"
+        "from fastcorticalwiring_amd import _lib, meshes\n"
+        "V, F = meshes.icosphere(4)\n"
+        "B = np.random.default_rng(2).standard_normal((V.shape[0], 2))\n"
+        "X, st = _lib.host_factor_solve(V * 20.0, F.astype(np.int32), 0.5, 1.0, B, leaf_size=16)\n"
+        "print(int(st['nnodes']), hashlib.sha256(X.tobytes()).hexdigest())\n")
+    outs = []
+    for threads in ("1", "3", "8"):
+        env = dict(os.environ, OMP_NUM_THREADS=threads, PYTHONPATH=ROOT)
+        r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        outs.append(r.stdout.strip())
+    assert int(outs[0].split()[0]) > 64  # levels wide enough to spread over the threads
+    assert outs[0] == outs[1] == outs[2]
