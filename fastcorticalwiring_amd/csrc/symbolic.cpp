@@ -21,28 +21,40 @@ struct Graph {
     std::vector<int> ptr, adj;  // sorted, unique, no self loops
 };
 
+// Vertex adjacency (ascending, no duplicates, no self loops) by bucketing both directions of
+// every face edge per vertex and sorting each vertex's short list.
 Graph build_graph(int n, int m, const int* F) {
-    std::vector<int64_t> keys;
-    keys.reserve((size_t)m * 6);
-    for (int f = 0; f < m; ++f) {
+    std::vector<int> start(n + 1, 0);
+    for (int f = 0; f < m; ++f)
         for (int a = 0; a < 3; ++a) {
-            int u = F[3 * f + a], v = F[3 * f + (a + 1) % 3];
+            const int u = F[3 * f + a], v = F[3 * f + (a + 1) % 3];
             if (u == v) continue;
-            keys.push_back(((int64_t)u << 32) | (uint32_t)v);
-            keys.push_back(((int64_t)v << 32) | (uint32_t)u);
+            start[u + 1]++;
+            start[v + 1]++;
         }
-    }
-    std::sort(keys.begin(), keys.end());
-    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    for (int i = 0; i < n; ++i) start[i + 1] += start[i];
+    std::vector<int> nb(start[n]);
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int f = 0; f < m; ++f)
+        for (int a = 0; a < 3; ++a) {
+            const int u = F[3 * f + a], v = F[3 * f + (a + 1) % 3];
+            if (u == v) continue;
+            nb[fill[u]++] = v;
+            nb[fill[v]++] = u;
+        }
     Graph g;
     g.ptr.assign(n + 1, 0);
-    g.adj.resize(keys.size());
-    for (size_t i = 0; i < keys.size(); ++i) {
-        int u = (int)(keys[i] >> 32);
-        g.ptr[u + 1]++;
-        g.adj[i] = (int)(keys[i] & 0xffffffff);
+    g.adj.resize(nb.size());
+    int64_t out = 0;
+    for (int u = 0; u < n; ++u) {
+        int* lo = nb.data() + start[u];
+        int* hi = nb.data() + start[u + 1];
+        std::sort(lo, hi);
+        hi = std::unique(lo, hi);
+        for (int* q = lo; q < hi; ++q) g.adj[out++] = *q;
+        g.ptr[u + 1] = (int)out;
     }
-    for (int i = 0; i < n; ++i) g.ptr[i + 1] += g.ptr[i];
+    g.adj.resize(out);
     return g;
 }
 
