@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -254,8 +255,12 @@ int factor(const Symbolic& S, const double* lval, const double* mdiag, double l_
 #pragma omp for schedule(dynamic, 1)
                 for (int i = lo; i < hi; ++i) {
                     const int id = S.level_nodes[i];
-                    int col = -1;
-                    const int rc = factor_node(S, id, lval, mdiag, l_scale, m_scale, panel, upd, front, false, &col);
+                    int col = -1, rc;
+                    try {  // no exception may leave the parallel region
+                        rc = factor_node(S, id, lval, mdiag, l_scale, m_scale, panel, upd, front, false, &col);
+                    } catch (const std::bad_alloc&) {
+                        rc = 3;
+                    }
                     if (rc) {
 #pragma omp critical(fcw_factor_fail)
                         report(id, rc, col);
@@ -276,6 +281,8 @@ int factor(const Symbolic& S, const double* lval, const double* mdiag, double l_
         if (fail_id >= 0) {
             if (fail_rc == 1)
                 err = "pattern entry outside front";
+            else if (fail_rc == 3)
+                err = "out of host memory for a front (node " + std::to_string(fail_id) + ")";
             else
                 err = "matrix not positive definite (node " + std::to_string(fail_id) + ", column " +
                       std::to_string(fail_col) + ")";
