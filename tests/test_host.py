@@ -239,3 +239,20 @@ def test_host_factorisation_independent_of_thread_count():
         outs.append(r.stdout.strip())
     assert int(outs[0].split()[0]) > 64  # levels wide enough to spread over the threads
     assert outs[0] == outs[1] == outs[2]
+
+
+def test_library_load_sets_passive_openmp_waits_unless_set():
+    """libgomp reads OMP_WAIT_POLICY when the library loads it: the loader defaults it to PASSIVE
+    (spinning waiters slow the factorisation ten-fold when other work shares the cores) and keeps
+    a caller's value."""
+    script = ("import os\nfrom fastcorticalwiring_amd import _lib\n_lib.lib()\n"
+              "print(os.environ.get('OMP_WAIT_POLICY'))\n")
+    env = {k: v for k, v in os.environ.items() if k != "OMP_WAIT_POLICY"}
+    env["PYTHONPATH"] = ROOT
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "PASSIVE"
+    env["OMP_WAIT_POLICY"] = "ACTIVE"
+    r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "ACTIVE"
