@@ -883,31 +883,42 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         };
         int n_eval = 0;
         double result = r, Pres = 0.0;
-        bool converged = false;
+        bool converged = false, have_p = false;
+        // When a Newton step starts within 1e-5 T of the target, the next iterate almost always
+        // meets the tolerance (quadratic convergence), so that evaluation also sums the perimeter
+        // and the separate perimeter pass at the root is skipped.
+        bool want_p = false;
         for (int it = 0; it < md.max_iter; ++it) {
             double A, dA, Pv;
-            eval(r, false, A, dA, Pv);
+            eval(r, want_p, A, dA, Pv);
             ++n_eval;
             const double err = A - T;
             if (fabs(err) <= md.rtol * T) {
                 result = r;
                 converged = true;
+                have_p = want_p;
+                Pres = Pv;
                 break;
             }
             if (err > 0.0) r_hi = r; else r_lo = r;
             double rn = 0.5 * (r_lo + r_hi);
+            bool newton_step = false;
             if (dA > 0.0) {
                 const double cand = r - err / dA;
-                if (r_lo < cand && cand < r_hi) rn = cand;
+                if (r_lo < cand && cand < r_hi) {
+                    rn = cand;
+                    newton_step = true;
+                }
             }
             if (r_hi - r_lo <= 1e-13 * (1.0 + r_hi)) {
                 result = rn;
                 break;
             }
+            want_p = newton_step && fabs(err) <= 1e-5 * T;
             r = rn;
             result = r;
         }
-        {
+        if (!have_p) {
             // perimeter at the root (counted as an evaluation when the root was not the last one)
             double A, dA;
             eval(result, true, A, dA, Pres);
