@@ -8,8 +8,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fcw_internal.h"
@@ -88,9 +90,63 @@ class Dissector {
         return id;
     }
 
+    // As build(), with the two halves of the top `par_depth` dissections built on their own
+    // threads, each by a Dissector with its own marks; their trees are appended A then B, which
+    // gives every node the id the sequential recursion gives it, so the tree is the same.
+    int build_parallel(std::vector<int> verts, int par_depth) {
+        if (par_depth <= 0) return build(std::move(verts));
+        int id = (int)tree.size();
+        tree.emplace_back();
+        if ((int)verts.size() <= leaf_) {
+            tree[id].verts = bfs_order(verts);
+            return id;
+        }
+        std::vector<int> A, B, sep;
+        split(verts, A, B, sep);
+        if (A.empty() || B.empty()) {
+            tree[id].verts = bfs_order(verts);
+            return id;
+        }
+        std::vector<int>().swap(verts);
+        const int n = (int)mark_.size();
+        Dissector DA(g_, n, leaf_), DB(g_, n, leaf_);
+        std::exception_ptr err_b;
+        std::thread tb([&] {
+            try {
+                DB.build_parallel(std::move(B), par_depth - 1);
+            } catch (...) {
+                err_b = std::current_exception();
+            }
+        });
+        std::exception_ptr err_a;
+        try {
+            DA.build_parallel(std::move(A), par_depth - 1);
+        } catch (...) {
+            err_a = std::current_exception();
+        }
+        tb.join();
+        if (err_a) std::rethrow_exception(err_a);
+        if (err_b) std::rethrow_exception(err_b);
+        const int ca = append(DA.tree);
+        const int cb = append(DB.tree);
+        tree[id].children = {ca, cb};
+        tree[id].verts = std::move(sep);
+        return id;
+    }
+
     std::vector<TreeNode> tree;
 
    private:
+    // Appends a subtree built by another Dissector (its root is its node 0); returns the root's id.
+    int append(std::vector<TreeNode>& sub) {
+        const int off = (int)tree.size();
+        for (TreeNode& t : sub) {
+            for (int& c : t.children) c += off;
+            tree.push_back(std::move(t));
+        }
+        return off;
+    }
+
     const Graph& g_;
     int leaf_;
     std::vector<int> mark_, lev_;
@@ -263,7 +319,7 @@ int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic
     {
         std::vector<int> all(n);
         std::iota(all.begin(), all.end(), 0);
-        D.build(std::move(all));
+        D.build_parallel(std::move(all), n >= 20000 ? 2 : 0);
     }
     std::vector<TreeNode>& T = D.tree;
     // Postorder numbering of tree nodes; positions assigned in postorder.

@@ -256,3 +256,21 @@ def test_library_load_sets_passive_openmp_waits_unless_set():
     r = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.strip() == "ACTIVE"
+
+
+def test_host_factorisation_with_threaded_dissection():
+    """Meshes of 20,000 vertices and more build the top two dissection levels on threads: the
+    factorisation must still solve the system exactly."""
+    from fastcorticalwiring_amd import _lib, meshes
+    from oracle import heat as oh
+
+    V, F = meshes.icosphere(6)
+    V = V * 50.0
+    F = F.astype(np.int32)
+    assert V.shape[0] >= 20000
+    L, M, *_ = oh.consistent_operators(V, F)
+    B = np.random.default_rng(3).standard_normal((V.shape[0], 1))
+    X, st = _lib.host_factor_solve(V, F, 0.7, 1.0, B)
+    Xr = spsolve((M + 0.7 * L).tocsc(), B[:, 0])
+    assert np.max(np.abs(X[:, 0] - Xr)) < 1e-11 * np.max(np.abs(Xr))
+    assert st["nnodes"] > 500
