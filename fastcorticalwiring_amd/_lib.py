@@ -49,7 +49,8 @@ class Options(ctypes.Structure):
         ("sparse_rhs", ctypes.c_int),
         ("robust", ctypes.c_int),
         ("split_depth", ctypes.c_int),
-        ("reserved", ctypes.c_int * 2),
+        ("band_cap", ctypes.c_int),
+        ("reserved", ctypes.c_int * 1),
     ]
 
 
@@ -119,7 +120,7 @@ EXPORTED_SYMBOLS = (
     "fcw_default_options", "fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block",
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
-    "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep",
+    "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse",
 )
 
 _lib = None
@@ -166,12 +167,14 @@ def lib():
     L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     L.fcw_host_intrinsic.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp, _dp, _dp]
+    L.fcw_host_analyse.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip,
+                                   _dp, ctypes.POINTER(ctypes.c_uint64)]
     L.fcw_mesh_prep.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int,
                                 ctypes.c_int, ctypes.POINTER(MeshPrepOut)]
     for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
                  "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy",
-                 "fcw_host_intrinsic", "fcw_mesh_prep"):
+                 "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -225,6 +228,23 @@ def host_factor_solve(V, F, l_scale, m_scale, B, leaf_size=64):
                                       int(leaf_size), dptr(B), B.shape[1], dptr(X), dptr(stats)))
     return X, dict(nnodes=int(stats[0]), nlevels=int(stats[1]), factor_nnz=int(stats[2]),
                    panel_elems=int(stats[3]))
+
+
+def host_analyse(V, F, leaf_size=64, par_depth=-1, panel_hash=True):
+    """CPU-only view of the nested-dissection analysis: dict with perm, node_start, node_s, node_b,
+    the counts and (optionally) the FNV-1a hash of the FP64 panels of M + L."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    n = V.shape[0]
+    perm = np.empty(n, np.int32)
+    ns, nsz, nb = (np.empty(n, np.int32) for _ in range(3))
+    stats = np.zeros(4)
+    h = ctypes.c_uint64(0)
+    check(lib().fcw_host_analyse(dptr(V), n, iptr(F), F.shape[0], int(leaf_size), int(par_depth), iptr(perm),
+                                 iptr(ns), iptr(nsz), iptr(nb), dptr(stats), ctypes.byref(h) if panel_hash else None))
+    nn = int(stats[0])
+    return dict(perm=perm, node_start=ns[:nn], node_s=nsz[:nn], node_b=nb[:nn], nnodes=nn, nlevels=int(stats[1]),
+                factor_nnz=int(stats[2]), panel_elems=int(stats[3]), panel_hash=int(h.value) if panel_hash else None)
 
 
 def host_intrinsic(V, F, mollify=1e-6):

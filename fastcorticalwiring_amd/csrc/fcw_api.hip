@@ -649,11 +649,12 @@ void fcw_default_options(fcw_options* opt) {
     opt->leaf_size = 64;
     opt->host_threads = 0;
     opt->device = 0;
-    opt->ir_steps = 1;
+    opt->ir_steps = 2;
     opt->streams = 8;
     opt->robust = 0;
     opt->sparse_rhs = 1;
     opt->split_depth = 0;
+    opt->band_cap = 6144;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -691,6 +692,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->n = n;
         C->m = m;
         C->ldk = ((opt.block + fcw::kColPad - 1) / fcw::kColPad) * fcw::kColPad;
+        if (opt.band_cap < 0) throw Fail("band_cap must be >= 0 (0 = default)");
+        if (opt.band_cap > 0) C->band_cap = opt.band_cap;
         // the divergence addresses one RHS block with 32-bit byte offsets
         if ((uint64_t)n * (uint64_t)C->ldk * 8u >= (1ull << 32))
             throw Fail("an RHS block of this mesh would exceed 4 GiB; use a smaller block");
@@ -1190,6 +1193,68 @@ int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double molli
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Stiffness values in the analysis's CSR order and the lumped mass in positions, assembled on the
+// host from the input surface.
+void host_assemble(const fcw::Symbolic& S, const double* V, const int* F, std::vector<double>& lval,
+                   std::vector<double>& mpos) {
+    const int n = S.n;
+    std::vector<std::vector<std::pair<int, double>>> rows;
+    std::vector<double> mass;
+    host_operators(n, V, S.m, F, rows, mass);
+    lval.assign(S.rowptr[n], 0.0);
+    mpos.assign(n, 0.0);
+    for (int v = 0; v < n; ++v) {
+        const int p = S.iperm[v];
+        mpos[p] = mass[v];
+        const int* row = S.colind.data() + S.rowptr[p];
+        const int len = S.rowptr[p + 1] - S.rowptr[p];
+        for (auto& e : rows[v]) {
+            const int qv = S.iperm[e.first];
+            const int pos = (int)(std::lower_bound(row, row + len, qv) - row);
+            lval[S.rowptr[p] + pos] += e.second;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int fcw_host_analyse(const double* V, int n, const int* F, int m, int leaf_size, int par_depth, int* perm,
+                     int* node_start, int* node_s, int* node_b, double* stats, uint64_t* panel_hash) {
+    return guard([&] {
+        if (!V || !F || !perm || !node_start || !node_s || !node_b) throw Fail("null argument");
+        check_faces(n, V, m, F);
+        fcw::Symbolic S;
+        std::string err;
+        if (fcw::analyse(n, V, m, F, leaf_size > 0 ? leaf_size : 64, S, err, par_depth)) throw Fail(err);
+        std::copy(S.perm.begin(), S.perm.end(), perm);
+        std::copy(S.node_start.begin(), S.node_start.end(), node_start);
+        std::copy(S.node_s.begin(), S.node_s.end(), node_s);
+        std::copy(S.node_b.begin(), S.node_b.end(), node_b);
+        if (stats) {
+            stats[0] = S.nnodes;
+            stats[1] = S.nlevels;
+            stats[2] = (double)S.factor_nnz;
+            stats[3] = (double)S.panel_elems;
+        }
+        if (panel_hash) {
+            std::vector<double> lval, mpos;
+            host_assemble(S, V, F, lval, mpos);
+            std::vector<double> panel((size_t)S.panel_elems);
+            if (fcw::factor(S, lval.data(), mpos.data(), 1.0, 1.0, panel.data(), err, 0)) throw Fail(err);
+            uint64_t h = 1469598103934665603ull;  // FNV-1a over the panel bytes
+            const unsigned char* b = reinterpret_cast<const unsigned char*>(panel.data());
+            for (size_t i = 0; i < panel.size() * sizeof(double); ++i) h = (h ^ b[i]) * 1099511628211ull;
+            *panel_hash = h;
+        }
+    });
+}
+
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale, int leaf_size,
                           const double* B, int k, double* X, double* stats) {
     return guard([&] {
@@ -1197,21 +1262,8 @@ int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_
         fcw::Symbolic S;
         std::string err;
         if (fcw::analyse(n, V, m, F, leaf_size > 0 ? leaf_size : 64, S, err)) throw Fail(err);
-        std::vector<std::vector<std::pair<int, double>>> rows;
-        std::vector<double> mass;
-        host_operators(n, V, m, F, rows, mass);
-        std::vector<double> lval(S.rowptr[n], 0.0), mpos(n);
-        for (int v = 0; v < n; ++v) {
-            const int p = S.iperm[v];
-            mpos[p] = mass[v];
-            const int* row = S.colind.data() + S.rowptr[p];
-            const int len = S.rowptr[p + 1] - S.rowptr[p];
-            for (auto& e : rows[v]) {
-                const int qv = S.iperm[e.first];
-                const int pos = (int)(std::lower_bound(row, row + len, qv) - row);
-                lval[S.rowptr[p] + pos] += e.second;
-            }
-        }
+        std::vector<double> lval, mpos;
+        host_assemble(S, V, F, lval, mpos);
         std::vector<double> panel((size_t)S.panel_elems);
         if (fcw::factor(S, lval.data(), mpos.data(), l_scale, m_scale, panel.data(), err, 0)) throw Fail(err);
         std::vector<double> Xp((size_t)n * k);

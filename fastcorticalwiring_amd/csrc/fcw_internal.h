@@ -54,7 +54,10 @@ struct Symbolic {
     int max_front = 0;
 };
 
-int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err);
+// par_depth: dissection levels whose two halves are built on their own threads (the tree and the
+// numbering do not depend on it); < 0 selects 2 on meshes of 20,000 vertices or more, else 0.
+int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err,
+            int par_depth = -1);
 
 // Intrinsic Delaunay triangulation of the surface (V, F) after mollification with
 // eps = mollify * mean edge length (intrinsic.cpp): same vertices, flipped faces, and the length

@@ -302,7 +302,7 @@ class Dissector {
 
 }  // namespace
 
-int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err) {
+int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic& S, std::string& err, int par_depth) {
     (void)V;
     if (n <= 0 || m <= 0) { err = "empty mesh"; return 1; }
     for (int64_t i = 0; i < 3LL * m; ++i)
@@ -319,7 +319,8 @@ int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic
     {
         std::vector<int> all(n);
         std::iota(all.begin(), all.end(), 0);
-        D.build_parallel(std::move(all), n >= 20000 ? 2 : 0);
+        // the halves of the top two dissections on their own threads on large meshes (same result)
+        D.build_parallel(std::move(all), par_depth >= 0 ? par_depth : (n >= 20000 ? 2 : 0));
     }
     std::vector<TreeNode>& T = D.tree;
     // Postorder numbering of tree nodes; positions assigned in postorder.

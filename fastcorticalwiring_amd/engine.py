@@ -25,8 +25,8 @@ class HeatContext:
     """
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
-                 precision="fp64", ir_steps=1, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False,
-                 split_depth=0):
+                 precision="fp64", ir_steps=2, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False,
+                 split_depth=0, band_cap=0):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
         if V.ndim != 2 or V.shape[1] != 3:
@@ -49,6 +49,8 @@ class HeatContext:
         opt.sparse_rhs = 1 if sparse_rhs else 0
         opt.robust = 1 if robust else 0
         opt.split_depth = int(split_depth)
+        if band_cap:
+            opt.band_cap = int(band_cap)
         handle = ctypes.c_void_p()
         _lib.check(L.fcw_create(_lib.dptr(V), V.shape[0], _lib.iptr(F), F.shape[0], ctypes.byref(opt),
                                 ctypes.byref(handle)))

@@ -3,8 +3,9 @@
  * wiring-metric engine. Plain C types only; the caller owns every host buffer,
  * the library owns device memory per context. Every call returns 0 on success
  * and a nonzero status on failure, with the message in fcw_last_error()
- * (thread-local). One context = one HIP stream; calls on one context must be
- * serialised by the caller, contexts are independent.
+ * (thread-local). A context runs its RHS blocks on its own 1-16 HIP streams
+ * (fcw_options.streams) and synchronises them before each call returns; calls on
+ * one context must be serialised by the caller, contexts are independent.
  *
  * Vertex and face arrays describe the cortical submesh, as the reference hands
  * it to its distance engine (core_analysis.py:646-667).
@@ -51,7 +52,9 @@ typedef struct {
     int leaf_size;         /* nested-dissection leaf size; default 64 */
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
-    int ir_steps;          /* FCW_FP32_IR: residual + correction solves after the first FP32 solve; default 1 */
+    int ir_steps;          /* FCW_FP32_IR: residual + correction solves after the first FP32 solve; default 2 (one
+                              step leaves ~2e-7 of the largest distance, 6e-5 relative on the smallest
+                              distances to the boundary of the 164k-vertex surface) */
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces), 1..16;
                               default 8 (run concurrently when GPU_MAX_HW_QUEUES >= streams) */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */
@@ -60,7 +63,9 @@ typedef struct {
     int split_depth;       /* FP64 sweeps of the narrow upper tree levels split their depth over several
                               workgroups (lower latency per block, more work in all): 1 on, -1 off, 0 (default)
                               on when streams <= 2 */
-    int reserved[2];
+    int band_cap;          /* band faces per scale held per source by the metric kernel (default 6144); a band
+                              that does not fit is evaluated from every face instead (same result, slower) */
+    int reserved[1];
 } fcw_options;
 
 typedef struct {
@@ -153,6 +158,17 @@ int fcw_calibration_copy(int64_t bytes, int device);
  * nnodes, nlevels, factor_nnz, panel_elems. */
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale,
                           int leaf_size, const double* B, int k, double* X, double* stats);
+
+/* Host-only view of the analysis (no GPU): the nested-dissection ordering and supernodes that
+ * fcw_create builds, with the dissection's top `par_depth` levels split over threads (< 0: the
+ * default rule, 2 on meshes of 20,000 vertices or more, else 0). The result does not depend on
+ * par_depth. perm (n) = input vertex at each position; node_start / node_s / node_b (n each, the
+ * first stats[0] entries written) = each supernode's first position, own and boundary row counts;
+ * stats[0..3] = nnodes, nlevels, factor_nnz, panel_elems; panel_hash (may be NULL) = FNV-1a hash of
+ * the FP64 panels of M + L. Replaces no reference interface: the ordering lives inside CHOLMOD in
+ * the reference's solver (fastcw_lean.py:457-464). */
+int fcw_host_analyse(const double* V, int n, const int* F, int m, int leaf_size, int par_depth, int* perm,
+                     int* node_start, int* node_s, int* node_b, double* stats, uint64_t* panel_hash);
 
 /* Mesh preparation on the device, the steps before the solves (core_analysis.py:651-683, 859-978,
  * 1336-1417, which build_cortex_submesh, boundary, face geometry, adjacency and

@@ -1,0 +1,180 @@
+"""GPU parity on the folded workloads against the FP64 oracle.
+
+The surfaces are the folded icospheres of level 5, 6 and 7 (10,242 / 40,962 /
+163,842 vertices, spherical-harmonic folding, 10% masked cap; level 7 also
+jittered). The oracle's outputs for 256 seeded cortical sources of each are
+fixtures (``tests/golden/make_oracle_fixtures.py``: SuperLU FP64 heat method and
+the reference loop restated, fastcw_lean.py:467-534 and
+core_analysis.py:1441-1632), since the level-7 oracle needs minutes of CPU time.
+
+Tolerances:
+* FP64 form: distances within 1e-9 relative (max |d - d_ref| / max d_ref, and
+  relative where d_ref > 5h as compare_fastcw_lean.py:235-239); radii,
+  perimeters and supplementary samples within 1e-8; MSD and distance to boundary
+  within 1e-6 (the fixture holds them as float32, as the reference's arrays do).
+* FP32+IR form: every output within 1e-5 relative, the hot path's target.
+* Flags and cleaned-sample counts are equal; sample values within 1e-5.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+pytestmark = pytest.mark.gpu
+
+TOL = {
+    "fp64": dict(dist=1e-9, geo=1e-8, scalar=1e-6, sample=1e-5),
+    "fp32_ir": dict(dist=1e-5, geo=1e-5, scalar=1e-5, sample=1e-5),
+}
+
+_cache = {}
+
+
+def folded(idx):
+    """(fixture, oracle mesh) of folded workload idx, the surface checked against the fixture's hash."""
+    if idx not in _cache:
+        import make_oracle_fixtures as mk
+
+        from fastcorticalwiring_amd import meshes
+        from oracle import analysis as oa
+
+        fx = mk.load(idx)
+        s = meshes.config_surface(idx)
+        mesh = oa.OracleMesh(s.vertices, s.faces, s.cortex_mask)
+        assert mk.submesh_sha(mesh.V, mesh.F) == bytes(fx["mesh_sha"]).hex(), "surface differs from the fixture's"
+        _cache[idx] = (fx, mesh, s)
+    return _cache[idx]
+
+
+def set_metrics(ctx, fx, mesh):
+    ctx.set_metrics(fx["targets"], fx["fracs"], mesh.boundary, mesh.vertex_areas, eps=1e-6,
+                    health_h=float(fx["health_h"]), bin_width=float(fx["width"]),
+                    boundary_cap_fraction=float(fx["cap"]))
+
+
+def check_distances(D, fx, tol):
+    Dp = D[:, fx["probes"]]
+    R = fx["D_probe"]
+    err = float(np.max(np.abs(Dp - R)) / np.max(R))
+    far = R > 5.0 * float(fx["health_h"])
+    rel = float(np.max(rel_err(Dp[far], R[far])))
+    assert np.all(D[np.arange(D.shape[0]), fx["src"]] == 0.0)
+    return err, rel
+
+
+def check_outputs(sc, radii, perim, extra_r, extra_a, sample_r, sample_a, fx, tol, tag=""):
+    """Compare one source set's outputs (rows in fixture source order) with the fixture."""
+    assert np.array_equal(sc[:, 0].astype(np.uint8), fx["flag"]), tag
+    ok = fx["flag"] == 0
+    worst = {}
+    for key, a, b in (("radii", radii, fx["radii"]), ("perimeters", perim, fx["perimeters"]),
+                      ("extra_r", extra_r, fx["extra_r"]), ("extra_a", extra_a, fx["extra_a"])):
+        a, b = a[ok], b[ok]
+        assert np.array_equal(np.isnan(a), np.isnan(b)), (tag, key)
+        fin = np.isfinite(b)
+        worst[key] = float(np.max(rel_err(a[fin], b[fin]), initial=0.0))
+        assert worst[key] < tol["geo"], (tag, key, worst[key])
+    for col, key in ((3, "dist_to_boundary"), (4, "msd_unweighted"), (5, "msd_weighted")):
+        worst[key] = float(np.max(rel_err(sc[ok, col], fx[key][ok].astype(np.float64)), initial=0.0))
+        assert worst[key] < tol["scalar"], (tag, key, worst[key])
+    ip = fx["sample_indptr"]
+    for row in np.flatnonzero(ok):
+        cnt = int(sc[row, 7])
+        lo, hi = ip[row], ip[row + 1]
+        assert cnt == hi - lo, (tag, row, cnt, hi - lo)
+        er = max(np.max(rel_err(sample_r[row, :cnt], fx["sample_r"][lo:hi]), initial=0.0),
+                 np.max(rel_err(sample_a[row, :cnt], fx["sample_a"][lo:hi]), initial=0.0))
+        worst["samples"] = max(worst.get("samples", 0.0), float(er))
+    assert worst.get("samples", 0.0) < tol["sample"], (tag, worst)
+    return worst
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("idx", [1, 2, 3])
+def test_folded_workload_matches_oracle(idx, precision):
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(idx)
+    tol = TOL[precision]
+    print(f"folded workload {idx} ({mesh.n} cortical vertices), {precision}", flush=True)
+    with HeatContext(mesh.V, mesh.F, block=256, precision=precision) as ctx:
+        assert abs(ctx.t - float(fx["t"])) <= 1e-12 * float(fx["t"])
+        D = ctx.distances(fx["src"])
+        err, rel = check_distances(D, fx, tol)
+        print(f"  distances: max abs/max {err:.2e}, rel(d>5h) {rel:.2e}", flush=True)
+        assert err < tol["dist"] and rel < tol["dist"], (err, rel)
+        set_metrics(ctx, fx, mesh)
+        w = ctx.wiring(fx["src"])
+    worst = check_outputs(w["scalars"], w["radii"], w["perimeters"], w["extra_r"], w["extra_a"], w["sample_r"],
+                          w["sample_a"], fx, tol, f"c{idx} {precision}")
+    print("  worst relative:", {k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
+
+
+def test_fp32_ir_matches_fp64_on_every_source_of_the_largest_surface():
+    """FP32+IR against the FP64 form (itself pinned to the oracle above) over all 148,165 cortical
+    sources of the 163,842-vertex surface: every metric within 1e-5 relative."""
+    from fastcorticalwiring_amd.engine import HeatContext
+    from fastcorticalwiring_amd import mesh_prep
+
+    fx, mesh, _ = folded(3)
+    order = np.asarray(mesh.bfs_order, dtype=np.int32)
+    out = {}
+    for precision in ("fp64", "fp32_ir"):
+        print("all sources,", precision, flush=True)
+        with HeatContext(mesh.V, mesh.F, block=256, precision=precision) as ctx:
+            set_metrics(ctx, fx, mesh)
+            out[precision] = ctx.wiring(order)
+    a, b = out["fp32_ir"], out["fp64"]
+    assert np.array_equal(a["scalars"][:, 0], b["scalars"][:, 0])
+    ok = b["scalars"][:, 0] == 0
+    assert ok.sum() == order.size
+    worst = {}
+    for key in ("radii", "perimeters", "extra_r", "extra_a"):
+        x, y = a[key][ok], b[key][ok]
+        assert np.array_equal(np.isnan(x), np.isnan(y)), key
+        fin = np.isfinite(y)
+        worst[key] = float(np.max(rel_err(x[fin], y[fin])))
+    for col, key in ((3, "dist_to_boundary"), (4, "msd_u"), (5, "msd_w")):
+        worst[key] = float(np.max(rel_err(a["scalars"][ok, col], b["scalars"][ok, col])))
+    print("fp32_ir vs fp64, all sources:", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    assert max(worst.values()) < 1e-5, worst
+
+
+def test_band_overflow_fallback_gives_the_same_metrics():
+    """A band list that does not fit the per-source pool is evaluated from every face instead
+    (the kernel's fallback); with a pool of 16 faces per scale most bands overflow, and the
+    results still match the oracle and the default pool's."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(1)
+    with HeatContext(mesh.V, mesh.F, block=256) as big, HeatContext(mesh.V, mesh.F, block=256, band_cap=16) as small:
+        set_metrics(big, fx, mesh)
+        set_metrics(small, fx, mesh)
+        a = big.wiring(fx["src"])
+        b = small.wiring(fx["src"])
+    assert np.all(a["scalars"][:, 8] == 0)
+    assert np.mean(b["scalars"][:, 8] > 0) > 0.9  # the fallback ran on (nearly) every source
+    check_outputs(b["scalars"], b["radii"], b["perimeters"], b["extra_r"], b["extra_a"], b["sample_r"],
+                  b["sample_a"], fx, TOL["fp64"], "band_cap=16")
+    for key in ("radii", "perimeters", "extra_r", "extra_a"):
+        x, y = a[key], b[key]
+        fin = np.isfinite(x)
+        assert np.array_equal(fin, np.isfinite(y))
+        assert np.max(rel_err(y[fin], x[fin])) < 1e-9, key
+    assert np.array_equal(a["scalars"][:, 7], b["scalars"][:, 7])
+
+
+def test_band_cap_option_is_validated():
+    from fastcorticalwiring_amd._lib import FcwError
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(1)
+    with pytest.raises(FcwError, match="band_cap"):
+        HeatContext(mesh.V, mesh.F, band_cap=-5)

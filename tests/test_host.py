@@ -274,3 +274,24 @@ def test_host_factorisation_with_threaded_dissection():
     Xr = spsolve((M + 0.7 * L).tocsc(), B[:, 0])
     assert np.max(np.abs(X[:, 0] - Xr)) < 1e-11 * np.max(np.abs(Xr))
     assert st["nnodes"] > 500
+
+
+def test_threaded_dissection_builds_the_same_tree():
+    """The threaded nested dissection (halves of the top dissections built on their own threads)
+    gives the same ordering, supernodes and panels as the sequential recursion, on a 163,842-vertex
+    icosphere where the default rule threads two levels."""
+    from fastcorticalwiring_amd import _lib, meshes
+
+    V, F = meshes.icosphere(7)
+    V = V * 70.0
+    F = F.astype(np.int32)
+    seq = _lib.host_analyse(V, F, par_depth=0)
+    for depth in (2, 3, -1):
+        par = _lib.host_analyse(V, F, par_depth=depth)
+        assert np.array_equal(seq["perm"], par["perm"]), depth
+        for key in ("node_start", "node_s", "node_b"):
+            assert np.array_equal(seq[key], par[key]), (depth, key)
+        assert (seq["nnodes"], seq["nlevels"], seq["factor_nnz"]) == (par["nnodes"], par["nlevels"],
+                                                                      par["factor_nnz"])
+        assert seq["panel_hash"] == par["panel_hash"], depth
+    assert sorted(seq["perm"].tolist()) == list(range(V.shape[0]))

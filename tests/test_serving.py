@@ -113,3 +113,34 @@ def test_results_alone_equal_results_in_the_mix():
             fin = np.isfinite(x)
             rel = np.abs(x[fin] - y[fin]) / np.maximum(np.abs(x[fin]), 1e-300)
             assert np.max(rel, initial=0.0) < 1e-9, key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_served_folded_surfaces_match_the_oracle(precision):
+    """One hemisphere each of icosphere level 5, 6 and 7 (the folded, capped workloads), arriving
+    together and served concurrently: every hemisphere's results at the oracle fixture's 256
+    sources match the FP64 oracle (FP64 form to 1e-8 on radii, perimeters and samples; FP32+IR to
+    1e-5 on every output)."""
+    import test_gpu_configs as tc
+
+    from fastcorticalwiring_amd.serving import HemisphereRequest, ServingEngine
+
+    reqs, fixtures = [], []
+    for i, idx in enumerate((1, 2, 3)):
+        fx, mesh, s = tc.folded(idx)
+        reqs.append(HemisphereRequest(i, 0.05 * i, s.vertices, s.faces, s.cortex_mask,
+                                      boundary_cap_fraction=float(fx["cap"])))
+        fixtures.append((fx, mesh))
+    res = ServingEngine(max_active=3, chunk_blocks=4, precision=precision).run(reqs)
+    for r, (fx, mesh) in zip(res, fixtures):
+        assert r.error is None, r.error
+        assert np.array_equal(r.sub_to_orig, mesh.sub_to_orig)
+        inv = np.empty(r.outputs["order"].size, np.int64)
+        inv[r.outputs["order"]] = np.arange(inv.size)
+        rows = inv[fx["src"]]
+        o = {k: v[rows] for k, v in r.outputs.items() if k != "order"}
+        worst = tc.check_outputs(o["scalars"], o["radii"], o["perimeters"], o["extra_r"], o["extra_a"],
+                                 o["sample_r"], o["sample_a"], fx, tc.TOL[precision], f"served c{int(fx['config'])}")
+        print(f"served level-{4 + int(fx['config'])} hemisphere ({mesh.n} sources), {precision}:",
+              {k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
