@@ -121,6 +121,7 @@ EXPORTED_SYMBOLS = (
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
     "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse",
+    "fcw_wiring_sources_noalloc",
 )
 
 _lib = None
@@ -156,6 +157,7 @@ def lib():
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int]
     L.fcw_wiring_sources.argtypes = [_vp, _ip, ctypes.c_int, ctypes.POINTER(MetricsOut)]
+    L.fcw_wiring_sources_noalloc.argtypes = [_vp, _ip, ctypes.c_int, ctypes.POINTER(MetricsOut)]
     L.fcw_geometry_fields.argtypes = [_vp, _dp, _ip, ctypes.c_int, ctypes.c_int, ctypes.POINTER(MetricsOut)]
     L.fcw_stage_times.argtypes = [_vp, _dp, ctypes.c_int]
     L.fcw_set_profiling.argtypes = [_vp, ctypes.c_int]
@@ -174,7 +176,7 @@ def lib():
     for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
                  "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy",
-                 "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse"):
+                 "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse", "fcw_wiring_sources_noalloc"):
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

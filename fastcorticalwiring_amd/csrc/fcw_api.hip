@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -83,7 +84,62 @@ enum TimedKernel { kFwdGemm = 0, kBwdGemm = 1, kMetrics = 2, kNumTimed = 3 };
 struct BlockPlan {
     std::vector<int> tile_off;  // per level, into the call's tile array
     std::vector<int> node_off;  // per level, into the call's node array
+    const fcw::LevelTile* tiles = nullptr;  // the call's tile and node arrays on the device
+    const int* nodes = nullptr;
 };
+
+// Host-side plans of one call's blocks, with the tile and node arrays to upload.
+struct CallPlans {
+    std::vector<BlockPlan> blocks;
+    std::vector<fcw::LevelTile> tiles;
+    std::vector<int> nodes;
+    void bind(const fcw::LevelTile* dtiles, const int* dnodes) {
+        for (auto& b : blocks) {
+            b.tiles = dtiles;
+            b.nodes = dnodes;
+        }
+    }
+};
+
+// Device output arrays of one call (rows follow the call's sources).
+struct OutBase {
+    double *scal = nullptr, *radii = nullptr, *perim = nullptr, *er = nullptr, *ea = nullptr;
+    float *sr = nullptr, *sa = nullptr;
+};
+
+}  // namespace
+
+// Device scratch of the allocation-free wiring entry point (fcw_wiring_sources_noalloc): the
+// call's source positions, sparse-sweep plans and device outputs. One copy per GPU, held until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (fcw_create resolves it once per device), never through this name.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr int kNoAllocMaxSources = 4096;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+struct ScratchSlot {
+    unsigned char* ptr = nullptr;  // device address of `scratch` on this GPU
+    std::mutex use;                // held by a call for as long as it uses the array
+};
+std::mutex g_scratch_init;
+ScratchSlot g_scratch[kMaxDevices];
+
+// The scratch array's address on `device`, which must be the current device.
+ScratchSlot& scratch_slot(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Fail("device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_init);
+    ScratchSlot& s = g_scratch[device];
+    if (!s.ptr) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Fail("hipGetSymbolAddress returned no address for the scratch array");
+        s.ptr = static_cast<unsigned char*>(p);
+    }
+    return s;
+}
 
 }  // namespace
 
@@ -109,6 +165,7 @@ struct fcw_ctx {
     fcw_info info{};
     int n = 0, m = 0, ldk = 0;
     int64_t dev_bytes = 0;
+    ScratchSlot* noalloc = nullptr;  // this device's scratch array (fcw_wiring_sources_noalloc)
     std::vector<std::unique_ptr<Workspace>> ws;
     std::vector<int> node_of_pos;
 
@@ -266,7 +323,7 @@ struct fcw_ctx {
             const fcw::LevelTile* tiles;
             int ntiles;
             if (plan) {
-                tiles = plan_tiles.p + plan->tile_off[l];
+                tiles = plan->tiles + plan->tile_off[l];
                 ntiles = plan->tile_off[l + 1] - plan->tile_off[l];
             } else {
                 tiles = fwd_tiles.p + fwd_tile_off[l];
@@ -286,7 +343,7 @@ struct fcw_ctx {
             ev_end(w, e);
         }
         if (plan)
-            fcw::launch_clear_rows(w.stream, plan_nodes.p + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
+            fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
     }
 
@@ -336,7 +393,7 @@ struct fcw_ctx {
             // the sparse forward sweep reads only the own rows of the plan's supernodes: those rows
             // of X are cleared (X holds the previous block's divergence), not the whole block
             const int L = sym.nlevels;
-            fcw::launch_clear_rows(w.stream, plan_nodes.p + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
+            fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, w.X.p, nullptr);
             fcw::launch_set_delta(w.stream, w.X.p, ldk, w.src_pos.p, k);
             solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p);
@@ -363,13 +420,16 @@ struct fcw_ctx {
         }
     }
 
-    // Host plans for consecutive blocks of `pos` (source positions), uploaded once.
-    std::vector<BlockPlan> make_plans(const std::vector<int>& pos) {
+    // Host plans for consecutive blocks of `pos` (source positions); the caller uploads the tile and
+    // node arrays and binds their device copies.
+    CallPlans plan_blocks(const std::vector<int>& pos) {
         const int L = sym.nlevels, nn = sym.nnodes;
         const int nblocks = ((int)pos.size() + ldk - 1) / ldk;
-        std::vector<BlockPlan> plans(nblocks);
-        std::vector<fcw::LevelTile> tiles;
-        std::vector<int> nodes;
+        CallPlans cp;
+        std::vector<BlockPlan>& plans = cp.blocks;
+        plans.resize(nblocks);
+        std::vector<fcw::LevelTile>& tiles = cp.tiles;
+        std::vector<int>& nodes = cp.nodes;
         std::vector<int> mark(nn, -1);
         std::vector<std::vector<int>> per_level(L);
         for (int b = 0; b < nblocks; ++b) {
@@ -395,9 +455,16 @@ struct fcw_ctx {
                 P.node_off.push_back((int)nodes.size());
             }
         }
-        plan_tiles.upload(tiles.data(), std::max<size_t>(1, tiles.size()), dev_bytes);
-        plan_nodes.upload(nodes.data(), std::max<size_t>(1, nodes.size()), dev_bytes);
-        return plans;
+        return cp;
+    }
+
+    // Plans uploaded to the context's own plan arrays.
+    CallPlans make_plans(const std::vector<int>& pos) {
+        CallPlans cp = plan_blocks(pos);
+        plan_tiles.upload(cp.tiles.data(), std::max<size_t>(1, cp.tiles.size()), dev_bytes);
+        plan_nodes.upload(cp.nodes.data(), std::max<size_t>(1, cp.nodes.size()), dev_bytes);
+        cp.bind(plan_tiles.p, plan_nodes.p);
+        return cp;
     }
 
     void ensure_outputs(size_t k) {
@@ -440,16 +507,55 @@ struct fcw_ctx {
         return md;
     }
 
-    fcw::MetricsOut outputs_at(size_t row) {
+    OutBase own_outputs() const {
+        OutBase b;
+        b.scal = o_scal.p;
+        b.radii = o_radii.p;
+        b.perim = o_perim.p;
+        b.er = o_er.p;
+        b.ea = o_ea.p;
+        b.sr = o_sr.p;
+        b.sa = o_sa.p;
+        return b;
+    }
+
+    // Device bytes of the outputs of k sources, laid out as carve_outputs places them.
+    size_t output_bytes(size_t k) const {
+        const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        return up(k * fcw::kSlotCount * 8) + 2 * up(k * S * 8) + 2 * up(std::max<size_t>(1, k * NE) * 8) +
+               2 * up(k * (S + NE) * 4);
+    }
+    OutBase carve_outputs(unsigned char* base, size_t k) const {
+        const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        OutBase b;
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            unsigned char* p = base + off;
+            off += up(bytes);
+            return p;
+        };
+        b.scal = reinterpret_cast<double*>(take(k * fcw::kSlotCount * 8));
+        b.radii = reinterpret_cast<double*>(take(k * S * 8));
+        b.perim = reinterpret_cast<double*>(take(k * S * 8));
+        b.er = reinterpret_cast<double*>(take(std::max<size_t>(1, k * NE) * 8));
+        b.ea = reinterpret_cast<double*>(take(std::max<size_t>(1, k * NE) * 8));
+        b.sr = reinterpret_cast<float*>(take(k * (S + NE) * 4));
+        b.sa = reinterpret_cast<float*>(take(k * (S + NE) * 4));
+        return b;
+    }
+
+    fcw::MetricsOut outputs_at(const OutBase& b, size_t row) const {
         const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
         fcw::MetricsOut o;
-        o.scal = o_scal.p + row * fcw::kSlotCount;
-        o.radii = o_radii.p + row * S;
-        o.perim = o_perim.p + row * S;
-        o.extra_r = o_er.p + row * NE;
-        o.extra_a = o_ea.p + row * NE;
-        o.samp_r = o_sr.p + row * (S + NE);
-        o.samp_a = o_sa.p + row * (S + NE);
+        o.scal = b.scal + row * fcw::kSlotCount;
+        o.radii = b.radii + row * S;
+        o.perim = b.perim + row * S;
+        o.extra_r = b.er + row * NE;
+        o.extra_a = b.ea + row * NE;
+        o.samp_r = b.sr + row * (S + NE);
+        o.samp_a = b.sa + row * (S + NE);
         return o;
     }
 
@@ -457,19 +563,60 @@ struct fcw_ctx {
         for (auto& w : ws) HIPCHK(hipStreamSynchronize(w->stream));
     }
 
-    void copy_outputs(size_t k, fcw_metrics_out* out) {
+    void copy_outputs(const OutBase& b, size_t k, fcw_metrics_out* out) {
         sync_all();
         const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
         auto cp = [&](void* dst, const void* src, size_t bytes) {
             if (dst && bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
         };
-        cp(out->scalars, o_scal.p, k * fcw::kSlotCount * sizeof(double));
-        cp(out->radii, o_radii.p, k * S * sizeof(double));
-        cp(out->perimeters, o_perim.p, k * S * sizeof(double));
-        cp(out->extra_r, o_er.p, k * NE * sizeof(double));
-        cp(out->extra_a, o_ea.p, k * NE * sizeof(double));
-        cp(out->sample_r, o_sr.p, k * (S + NE) * sizeof(float));
-        cp(out->sample_a, o_sa.p, k * (S + NE) * sizeof(float));
+        cp(out->scalars, b.scal, k * fcw::kSlotCount * sizeof(double));
+        cp(out->radii, b.radii, k * S * sizeof(double));
+        cp(out->perimeters, b.perim, k * S * sizeof(double));
+        cp(out->extra_r, b.er, k * NE * sizeof(double));
+        cp(out->extra_a, b.ea, k * NE * sizeof(double));
+        cp(out->sample_r, b.sr, k * (S + NE) * sizeof(float));
+        cp(out->sample_a, b.sa, k * (S + NE) * sizeof(float));
+    }
+
+    // The blocks of one wiring call: source positions `dpos` (device, nsrc), plans bound to device
+    // arrays, outputs at `ob`. Issues every block on the workspaces' streams; no allocation here
+    // (profiling events aside, which only the timing mode creates).
+    void run_wiring(const int* dpos, int nsrc, const CallPlans& plans, const OutBase& ob) {
+        const fcw::MeshDev mdv = mesh_dev();
+        const int NS = (int)ws.size();
+        for (int b0 = 0, bi = 0; b0 < nsrc; b0 += ldk, ++bi) {
+            Workspace& w = *ws[bi % NS];
+            const int k = std::min(ldk, nsrc - b0);
+            HIPCHK(hipMemcpyAsync(w.src_pos.p, dpos + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, w.stream));
+            distances(w, k, opt.sparse_rhs ? &plans.blocks[bi] : nullptr);
+            EventPair* e = ev_begin(w, kMetrics);
+            hipEvent_t ma = nullptr, mb = nullptr;
+            if (profiling) {
+                HIPCHK(hipEventCreate(&ma));
+                HIPCHK(hipEventCreate(&mb));
+                HIPCHK(hipEventRecord(ma, w.stream));
+            }
+            fcw::launch_chunk_stats(w.stream, k, ldk, nch, w.Z.p, w.src_pos.p, V.p, vw.p, isb.p, cvptr.p, cverts.p,
+                                    cnown.p, eps, health_h, w.ch_lo.p, w.ch_hi.p, w.cpart.p);
+            fcw::MetricsDev md = metrics_dev(w);
+            md.ch_lo = w.ch_lo.p;
+            md.ch_hi = w.ch_hi.p;
+            md.part = w.cpart.p;
+            md.shift_src = w.Z.p;
+            fcw::launch_metrics(w.stream, k, mdv, md, w.Dt.p, w.src_pos.p, outputs_at(ob, (size_t)b0), true);
+            ev_end(w, e);
+            HIPCHK(hipGetLastError());
+            if (profiling) {
+                HIPCHK(hipEventRecord(mb, w.stream));
+                HIPCHK(hipEventSynchronize(mb));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ma, mb));
+                stage[3] += ms * 1e-3;
+                (void)hipEventDestroy(ma);
+                (void)hipEventDestroy(mb);
+                ev_collect(w);
+            }
+        }
     }
 };
 
@@ -698,6 +845,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         if ((uint64_t)n * (uint64_t)C->ldk * 8u >= (1ull << 32))
             throw Fail("an RHS block of this mesh would exceed 4 GiB; use a smaller block");
         HIPCHK(hipSetDevice(opt.device));
+        C->noalloc = &scratch_slot(opt.device);
         double t0 = now_s();
         std::string err;
         fcw::Symbolic& S = C->sym;
@@ -966,8 +1114,8 @@ int fcw_distance_block(fcw_ctx* C, const int* src, int k, double* out) {
         std::vector<int> pos = source_positions(C, src, k);
         Workspace& w = *C->ws[0];
         HIPCHK(hipMemcpy(w.src_pos.p, pos.data(), sizeof(int) * k, hipMemcpyHostToDevice));
-        std::vector<BlockPlan> plans = C->make_plans(pos);
-        C->distances(w, k, C->opt.sparse_rhs ? &plans[0] : nullptr);
+        CallPlans plans = C->make_plans(pos);
+        C->distances(w, k, C->opt.sparse_rhs ? &plans.blocks[0] : nullptr);
         if (!C->Dout.p) C->Dout.alloc((size_t)C->n * C->ldk, C->dev_bytes);
         fcw::launch_gather_rows(w.stream, C->n, C->ldk, k, w.Dt.p, w.Z.p, w.src_pos.p, C->iperm.p, C->Dout.p);
         HIPCHK(hipGetLastError());
@@ -1038,47 +1186,58 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
         DBuf<int> all_pos;
         int64_t scratch_bytes = 0;
         all_pos.upload(pos.data(), pos.size(), scratch_bytes);
-        std::vector<BlockPlan> plans = C->make_plans(pos);
+        CallPlans plans = C->make_plans(pos);
         C->ensure_outputs((size_t)nsrc);
-        const fcw::MeshDev mdv = C->mesh_dev();
-        const int NS = (int)C->ws.size();
-        for (int b0 = 0, bi = 0; b0 < nsrc; b0 += C->ldk, ++bi) {
-            Workspace& w = *C->ws[bi % NS];
-            const int k = std::min(C->ldk, nsrc - b0);
-            HIPCHK(hipMemcpyAsync(w.src_pos.p, all_pos.p + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, w.stream));
-            C->distances(w, k, C->opt.sparse_rhs ? &plans[bi] : nullptr);
-            EventPair* e = C->ev_begin(w, kMetrics);
-            hipEvent_t ma = nullptr, mb = nullptr;
-            if (C->profiling) {
-                HIPCHK(hipEventCreate(&ma));
-                HIPCHK(hipEventCreate(&mb));
-                HIPCHK(hipEventRecord(ma, w.stream));
-            }
-            fcw::launch_chunk_stats(w.stream, k, C->ldk, C->nch, w.Z.p, w.src_pos.p, C->V.p, C->vw.p, C->isb.p,
-                                    C->cvptr.p, C->cverts.p, C->cnown.p, C->eps, C->health_h, w.ch_lo.p, w.ch_hi.p,
-                                    w.cpart.p);
-            fcw::MetricsDev md = C->metrics_dev(w);
-            md.ch_lo = w.ch_lo.p;
-            md.ch_hi = w.ch_hi.p;
-            md.part = w.cpart.p;
-            md.shift_src = w.Z.p;
-            fcw::launch_metrics(w.stream, k, mdv, md, w.Dt.p, w.src_pos.p, C->outputs_at((size_t)b0), true);
-            C->ev_end(w, e);
-            HIPCHK(hipGetLastError());
-            if (C->profiling) {
-                HIPCHK(hipEventRecord(mb, w.stream));
-                HIPCHK(hipEventSynchronize(mb));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ma, mb));
-                C->stage[3] += ms * 1e-3;
-                (void)hipEventDestroy(ma);
-                (void)hipEventDestroy(mb);
-                C->ev_collect(w);
-            }
-        }
+        C->run_wiring(all_pos.p, nsrc, plans, C->own_outputs());
         C->stage[5] = now_s() - w0;
-        C->copy_outputs((size_t)nsrc, out);
+        C->copy_outputs(C->own_outputs(), (size_t)nsrc, out);
         for (auto& w : C->ws) C->ev_collect(*w);
+        C->stage[4] = now_s() - w0;
+    });
+}
+
+int fcw_wiring_sources_noalloc(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* out) {
+    return guard([&] {
+        if (!C || !src || !out) throw Fail("null argument");
+        if (!C->metrics_ready) throw Fail("fcw_set_metrics must be called first");
+        if (nsrc <= 0) return;
+        if (nsrc > kNoAllocMaxSources)
+            throw Fail("fcw_wiring_sources_noalloc takes at most " + std::to_string(kNoAllocMaxSources) +
+                       " sources per call (got " + std::to_string(nsrc) + ")");
+        if (C->profiling) throw Fail("fcw_wiring_sources_noalloc does not run with profiling on (it creates events)");
+        const double w0 = now_s();
+        HIPCHK(hipSetDevice(C->opt.device));
+        std::vector<int> pos = source_positions(C, src, nsrc);
+        CallPlans plans = C->plan_blocks(pos);
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t b_pos = up(pos.size() * sizeof(int));
+        const size_t b_tiles = up(std::max<size_t>(1, plans.tiles.size()) * sizeof(fcw::LevelTile));
+        const size_t b_nodes = up(std::max<size_t>(1, plans.nodes.size()) * sizeof(int));
+        const size_t need = b_pos + b_tiles + b_nodes + C->output_bytes((size_t)nsrc);
+        if (need > kNoAllocScratchBytes)
+            throw Fail("these " + std::to_string(nsrc) + " sources need " + std::to_string(need) +
+                       " bytes of device scratch; fcw_wiring_sources_noalloc holds " +
+                       std::to_string(kNoAllocScratchBytes) + ": pass fewer sources per call");
+        // one call at a time uses this GPU's array; the previous call has synchronised its streams
+        std::lock_guard<std::mutex> lk(C->noalloc->use);
+        unsigned char* base = C->noalloc->ptr;
+        int* dpos = reinterpret_cast<int*>(base);
+        fcw::LevelTile* dtiles = reinterpret_cast<fcw::LevelTile*>(base + b_pos);
+        int* dnodes = reinterpret_cast<int*>(base + b_pos + b_tiles);
+        const OutBase ob = C->carve_outputs(base + b_pos + b_tiles + b_nodes, (size_t)nsrc);
+        hipStream_t st = C->ws[0]->stream;
+        HIPCHK(hipMemcpyAsync(dpos, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        if (!plans.tiles.empty())
+            HIPCHK(hipMemcpyAsync(dtiles, plans.tiles.data(), plans.tiles.size() * sizeof(fcw::LevelTile),
+                                  hipMemcpyHostToDevice, st));
+        if (!plans.nodes.empty())
+            HIPCHK(hipMemcpyAsync(dnodes, plans.nodes.data(), plans.nodes.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));  // every stream's blocks read these
+        plans.bind(dtiles, dnodes);
+        C->run_wiring(dpos, nsrc, plans, ob);
+        C->stage[5] = now_s() - w0;
+        C->copy_outputs(ob, (size_t)nsrc, out);
         C->stage[4] = now_s() - w0;
     });
 }
@@ -1099,10 +1258,10 @@ int fcw_geometry_fields(fcw_ctx* C, const double* D, const int* src, int k, int 
             for (int p = 0; p < C->n; ++p) Dp[(size_t)j * C->n + p] = D[(size_t)j * C->n + C->sym.perm[p]];
         HIPCHK(hipMemcpy(w.Dt.p, Dp.data(), sizeof(double) * Dp.size(), hipMemcpyHostToDevice));
         C->ensure_outputs((size_t)k);
-        fcw::launch_metrics(w.stream, k, C->mesh_dev(), C->metrics_dev(w), w.Dt.p, w.src_pos.p, C->outputs_at(0),
-                            health_screen != 0);
+        fcw::launch_metrics(w.stream, k, C->mesh_dev(), C->metrics_dev(w), w.Dt.p, w.src_pos.p,
+                            C->outputs_at(C->own_outputs(), 0), health_screen != 0);
         HIPCHK(hipGetLastError());
-        C->copy_outputs((size_t)k, out);
+        C->copy_outputs(C->own_outputs(), (size_t)k, out);
     });
 }
 

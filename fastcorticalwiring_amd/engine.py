@@ -140,6 +140,17 @@ class HeatContext:
             _lib.check(_lib.lib().fcw_wiring_sources(self._h, _lib.iptr(src), src.size, ctypes.byref(mo)))
         return arrs
 
+    def wiring_noalloc(self, sources):
+        """As wiring(), through the entry point that allocates no device memory while it runs
+        (fcw_wiring_sources_noalloc: at most 4096 sources per call)."""
+        if self._metrics is None:
+            raise RuntimeError("set_metrics() must be called first")
+        src = np.ascontiguousarray(np.asarray(sources, dtype=np.int64).reshape(-1), dtype=np.int32)
+        arrs, mo = self._alloc_out(src.size)
+        if src.size:
+            _lib.check(_lib.lib().fcw_wiring_sources_noalloc(self._h, _lib.iptr(src), src.size, ctypes.byref(mo)))
+        return arrs
+
     def geometry_fields(self, D, sources=None, health_screen=False):
         """Metrics of caller-supplied fields (k, n); the health screen only when asked."""
         if self._metrics is None:

@@ -124,6 +124,16 @@ int fcw_set_metrics(fcw_ctx* ctx, const double* targets, int S, const double* fr
  * perimeters, supplementary samples -> cleaned samples. Output rows follow `src`. */
 int fcw_wiring_sources(fcw_ctx* ctx, const int* src, int nsrc, fcw_metrics_out* out);
 
+/* As fcw_wiring_sources, with no device allocation in the call: for callers that cannot allocate
+ * device memory while they run (tight serving loops, code that must not call hipMalloc). It has no
+ * counterpart in the reference. The call's source positions, sweep plans and device outputs live
+ * in a static device array of 16 MiB per GPU (held until the process ends); at most 4096 sources
+ * per call, and fewer when the outputs per source are large (S scales, q samples: about
+ * 1,100 bytes per source at S = 12, q = 3), else the call fails with a message saying so. The
+ * context's own workspaces, created by fcw_create and fcw_set_metrics, are reused. Calls on one GPU
+ * share its array and are serialised by a lock; profiling must be off. */
+int fcw_wiring_sources_noalloc(fcw_ctx* ctx, const int* src, int nsrc, fcw_metrics_out* out);
+
 /* Metrics of caller-supplied fields D (k x n, input vertex order): the batched counterpart of
  * lean_geometry.per_source_geometry (lean_geometry.py:302-340) when health_screen = 0, and of the
  * whole per-source loop body on given fields (core_analysis.py:1558-1609) when health_screen = 1.
