@@ -121,7 +121,7 @@ EXPORTED_SYMBOLS = (
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
     "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse",
-    "fcw_wiring_sources_noalloc",
+    "fcw_wiring_sources_noalloc", "fcw_kernel_log",
 )
 
 _lib = None
@@ -162,6 +162,8 @@ def lib():
     L.fcw_stage_times.argtypes = [_vp, _dp, ctypes.c_int]
     L.fcw_set_profiling.argtypes = [_vp, ctypes.c_int]
     L.fcw_kernel_times.argtypes = [_vp, _dp, ctypes.c_int]
+    L.fcw_kernel_log.argtypes = [_vp, _dp, ctypes.c_int]
+    L.fcw_kernel_log.restype = ctypes.c_int
     L.fcw_build_info.argtypes = [ctypes.c_char_p, ctypes.c_int]
     L.fcw_last_error.argtypes = []
     L.fcw_last_error.restype = ctypes.c_char_p

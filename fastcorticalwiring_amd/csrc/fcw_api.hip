@@ -77,13 +77,34 @@ struct EventPair {
 };
 
 // Kernels timed per launch when profiling is on.
-enum TimedKernel { kFwdGemm = 0, kBwdGemm = 1, kMetrics = 2, kNumTimed = 3 };
+enum TimedKernel {
+    kFwdGemm = 0,  // FP64 forward sweep launch
+    kBwdGemm = 1,  // FP64 backward sweep launch
+    kMetrics = 2,  // metric kernel
+    kFwd32 = 3,    // FP32 forward sweep launch (refined solve)
+    kBwd32 = 4,    // FP32 backward sweep launch
+    kDivergence = 5,
+    kChunkStats = 6,
+    kResidual = 7,  // FP64 residual of the refined solve
+    kNumTimed = 8
+};
+
+// One timed launch: kind (TimedKernel), system (0 heat, 1 Poisson, -1 none), tree level (-1 none),
+// workgroups, factor entries of the supernodes it covers (sweeps: sum of s(s+1)/2 + s*b), and, once
+// collected, its duration.
+struct LaunchRec {
+    int kind, system, level, wgs;
+    double entries;
+    float ms;
+};
+constexpr size_t kMaxLaunchLog = 1 << 20;
 
 // Active-node plan of one block's heat forward sweep: only supernodes on the paths from the
 // block's sources to the root see a nonzero right-hand side.
 struct BlockPlan {
     std::vector<int> tile_off;  // per level, into the call's tile array
     std::vector<int> node_off;  // per level, into the call's node array
+    std::vector<double> level_entries;  // per level: factor entries of the active supernodes
     const fcw::LevelTile* tiles = nullptr;  // the call's tile and node arrays on the device
     const int* nodes = nullptr;
 };
@@ -155,7 +176,7 @@ struct Workspace {
     DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
     DBuf<int> src_pos;
     std::vector<EventPair> ev_pool;
-    std::vector<std::pair<int, EventPair>> ev_pending;
+    std::vector<std::pair<LaunchRec, EventPair>> ev_pending;
     size_t ev_used = 0;
 };
 
@@ -212,8 +233,10 @@ struct fcw_ctx {
     // timing
     double stage[6] = {0, 0, 0, 0, 0, 0};
     bool profiling = false;
-    double ktime[kNumTimed] = {0, 0, 0};
-    int64_t kcount[kNumTimed] = {0, 0, 0};
+    double ktime[kNumTimed] = {};
+    int64_t kcount[kNumTimed] = {};
+    std::vector<LaunchRec> launch_log;  // every timed launch while profiling (at most kMaxLaunchLog)
+    std::vector<double> level_entries;   // per tree level: sum over its supernodes of s(s+1)/2 + s*b
 
     ~fcw_ctx() {
         for (auto& w : ws) {
@@ -259,7 +282,7 @@ struct fcw_ctx {
         return d;
     }
 
-    EventPair* ev_begin(Workspace& w, int which) {
+    EventPair* ev_begin(Workspace& w, int which, int system = -1, int level = -1, int wgs = 0, double entries = 0.0) {
         if (!profiling) return nullptr;
         if (w.ev_used == w.ev_pool.size()) {
             EventPair e;
@@ -269,7 +292,7 @@ struct fcw_ctx {
         }
         EventPair* e = &w.ev_pool[w.ev_used++];
         HIPCHK(hipEventRecord(e->a, w.stream));
-        w.ev_pending.push_back({which, *e});
+        w.ev_pending.push_back({LaunchRec{which, system, level, wgs, entries, 0.f}, *e});
         return e;
     }
     void ev_end(Workspace& w, EventPair* e) {
@@ -281,8 +304,12 @@ struct fcw_ctx {
         for (auto& pe : w.ev_pending) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, pe.second.a, pe.second.b));
-            ktime[pe.first] += ms;
-            kcount[pe.first] += 1;
+            ktime[pe.first.kind] += ms;
+            kcount[pe.first.kind] += 1;
+            if (launch_log.size() < kMaxLaunchLog) {
+                pe.first.ms = ms;
+                launch_log.push_back(pe.first);
+            }
         }
         w.ev_pending.clear();
         w.ev_used = 0;
@@ -317,26 +344,40 @@ struct fcw_ctx {
     // One system: forward then backward sweep, X (RHS) -> Z (solution) through Y/U. With a plan the
     // forward sweep visits only the plan's active supernodes (Y/U must then be the heat pair, zero on
     // every other row) and their rows are cleared again after the backward sweep.
-    void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, double* Dt = nullptr) {
+    // Cholesky factor entries of one supernode: its triangle and its boundary rows.
+    double node_entries(int id) const {
+        const double s_ = sym.node_s[id], b_ = sym.node_b[id];
+        return s_ * (s_ + 1.0) / 2.0 + s_ * b_;
+    }
+
+    // Workgroups of a sweep launch over ntiles row tiles (the kernel's grid, one depth part).
+    int sweep_wgs(int ntiles) const { return (ntiles + 7) / 8 * 8 * (ldk / fcw::kTileN); }
+
+    void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, double* Dt = nullptr,
+               int system = 0) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
             const fcw::LevelTile* tiles;
             int ntiles;
+            double ent;
             if (plan) {
                 tiles = plan->tiles + plan->tile_off[l];
                 ntiles = plan->tile_off[l + 1] - plan->tile_off[l];
+                ent = plan->level_entries[l];
             } else {
                 tiles = fwd_tiles.p + fwd_tile_off[l];
                 ntiles = fwd_tile_off[l + 1] - fwd_tile_off[l];
+                ent = level_entries[l];
             }
-            EventPair* e = ev_begin(w, kFwdGemm);
+            EventPair* e = ntiles > 0 ? ev_begin(w, kFwdGemm, system, l, sweep_wgs(ntiles), ent) : nullptr;
             fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
                                  node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0,
                                  (int)sym.urows, plan ? 1 : fwd_split[l], w.part.p);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
-            EventPair* e = ev_begin(w, kBwdGemm);
+            EventPair* e = ev_begin(w, kBwdGemm, system, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l]),
+                                    level_entries[l]);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p,
                                  bwd_idx_lds, Dt, n, bwd_split[l], w.part.p);
@@ -352,14 +393,14 @@ struct fcw_ctx {
     void solve32(Workspace& w, const float* panel, double* acc64, int acc_mode, double* Dt = nullptr) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
-            EventPair* e = ev_begin(w, kFwdGemm);
+            EventPair* e = ev_begin(w, kFwd32, 1, l, sweep_wgs(fwd_tile_off[l + 1] - fwd_tile_off[l]), level_entries[l]);
             fcw::launch_fwd_gemm(w.stream, fwd_tiles.p + fwd_tile_off[l], fwd_tile_off[l + 1] - fwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, cown.p, ctail.p, panel,
                                  w.X32.p, w.Y32.p, w.U32.p, fwd_idx_lds, l == 0, (int)sym.urows);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
-            EventPair* e = ev_begin(w, kBwdGemm);
+            EventPair* e = ev_begin(w, kBwd32, 1, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l]), level_entries[l]);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
                                  w.Z32.p, bwd_idx_lds, acc64, acc_mode, Dt, n);
@@ -370,12 +411,16 @@ struct fcw_ctx {
     // Poisson system by refinement: an FP32 solve of the right-hand side, then ir_steps times the
     // FP64 residual r = b - A x rounded to FP32 and an FP32 solve of A dx = r added to x (FP64).
     void poisson_refined(Workspace& w) {
+        EventPair* e = ev_begin(w, kResidual, 1);
         fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p, nullptr,
                                w.X32.p);
+        ev_end(w, e);
         solve32(w, panel_pois32.p, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
+            e = ev_begin(w, kResidual, 1);
             fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p,
                                    w.Z.p, w.X32.p);
+            ev_end(w, e);
             solve32(w, panel_pois32.p, w.Z.p, 2, it + 1 == opt.ir_steps ? w.Dt.p : nullptr);
         }
     }
@@ -396,18 +441,20 @@ struct fcw_ctx {
             fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, w.X.p, nullptr);
             fcw::launch_set_delta(w.stream, w.X.p, ldk, w.src_pos.p, k);
-            solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p);
+            solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p, nullptr, 0);
         } else {
             fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
-            solve(w, panel_heat.p, nullptr, w.Yp.p, w.Up.p);
+            solve(w, panel_heat.p, nullptr, w.Yp.p, w.Up.p, nullptr, 0);
         }
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
+        EventPair* ed = ev_begin(w, kDivergence);
         fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p);
+        ev_end(w, ed);
         if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
         if (opt.precision == FCW_FP32_IR)
             poisson_refined(w);
         else
-            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p);
+            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p, 1);
         if (profiling) {
             HIPCHK(hipEventRecord(ev[3], w.stream));
             HIPCHK(hipEventSynchronize(ev[3]));
@@ -446,11 +493,14 @@ struct fcw_ctx {
             P.node_off.push_back((int)nodes.size());
             for (int l = 0; l < L; ++l) {
                 std::sort(per_level[l].begin(), per_level[l].end());
+                double ent = 0.0;
                 for (int id : per_level[l]) {
                     nodes.push_back(id);
                     const int f = sym.node_s[id] + sym.node_b[id];
                     for (int m0 = 0; m0 < f; m0 += fcw::kTileM) tiles.push_back({id, m0});
+                    ent += node_entries(id);
                 }
+                P.level_entries.push_back(ent);
                 P.tile_off.push_back((int)tiles.size());
                 P.node_off.push_back((int)nodes.size());
             }
@@ -589,15 +639,17 @@ struct fcw_ctx {
             const int k = std::min(ldk, nsrc - b0);
             HIPCHK(hipMemcpyAsync(w.src_pos.p, dpos + b0, sizeof(int) * k, hipMemcpyDeviceToDevice, w.stream));
             distances(w, k, opt.sparse_rhs ? &plans.blocks[bi] : nullptr);
-            EventPair* e = ev_begin(w, kMetrics);
             hipEvent_t ma = nullptr, mb = nullptr;
             if (profiling) {
                 HIPCHK(hipEventCreate(&ma));
                 HIPCHK(hipEventCreate(&mb));
                 HIPCHK(hipEventRecord(ma, w.stream));
             }
+            EventPair* ec = ev_begin(w, kChunkStats);
             fcw::launch_chunk_stats(w.stream, k, ldk, nch, w.Z.p, w.src_pos.p, V.p, vw.p, isb.p, cvptr.p, cverts.p,
                                     cnown.p, eps, health_h, w.ch_lo.p, w.ch_hi.p, w.cpart.p);
+            ev_end(w, ec);
+            EventPair* e = ev_begin(w, kMetrics);
             fcw::MetricsDev md = metrics_dev(w);
             md.ch_lo = w.ch_lo.p;
             md.ch_hi = w.ch_hi.p;
@@ -1019,6 +1071,9 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
             C->fwd_tile_off.push_back((int)ft.size());
             C->bwd_tile_off.push_back((int)bt.size());
         }
+        C->level_entries.assign(S.nlevels, 0.0);
+        for (int l = 0; l < S.nlevels; ++l)
+            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) C->level_entries[l] += C->node_entries(S.level_nodes[i]);
         C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
         C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
         C->fwd_split.assign(S.nlevels, 1);
@@ -1281,6 +1336,7 @@ int fcw_set_profiling(fcw_ctx* C, int on) {
             C->ktime[i] = 0.0;
             C->kcount[i] = 0;
         }
+        C->launch_log.clear();
         for (int i = 0; i < 6; ++i) C->stage[i] = 0.0;
     });
 }
@@ -1291,6 +1347,24 @@ int fcw_kernel_times(fcw_ctx* C, double* out, int len) {
     for (int i = 0; i < kNumTimed; ++i) {
         if (c < len) out[c++] = C->ktime[i];
         if (c < len) out[c++] = (double)C->kcount[i];
+    }
+    return c;
+}
+
+int fcw_kernel_log(fcw_ctx* C, double* out, int max_rows) {
+    if (!C) return 0;
+    const int n = (int)C->launch_log.size();
+    if (!out) return n;
+    const int c = std::min(n, max_rows);
+    for (int i = 0; i < c; ++i) {
+        const LaunchRec& r = C->launch_log[i];
+        double* o = out + 6 * (size_t)i;
+        o[0] = r.kind;
+        o[1] = r.system;
+        o[2] = r.level;
+        o[3] = r.wgs;
+        o[4] = r.entries;
+        o[5] = r.ms;
     }
     return c;
 }

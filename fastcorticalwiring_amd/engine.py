@@ -173,7 +173,25 @@ class HeatContext:
         return dict(heat_solve=out[0], fused_div=out[1], poisson_solve=out[2], metrics=out[3], wall=out[4],
                     issue=out[5])
 
+    KERNEL_KINDS = ("fwd_gemm", "bwd_gemm", "metrics", "fwd_gemm32", "bwd_gemm32", "divergence", "chunk_stats",
+                    "residual")
+
     def kernel_times(self):
-        out = np.zeros(6)
-        _lib.lib().fcw_kernel_times(self._h, _lib.dptr(out), 6)
-        return dict(fwd_gemm=(out[0], int(out[1])), bwd_gemm=(out[2], int(out[3])), metrics=(out[4], int(out[5])))
+        """{kind: (total ms, launches)} since profiling was switched on."""
+        k = len(self.KERNEL_KINDS)
+        out = np.zeros(2 * k)
+        _lib.lib().fcw_kernel_times(self._h, _lib.dptr(out), 2 * k)
+        return {name: (out[2 * i], int(out[2 * i + 1])) for i, name in enumerate(self.KERNEL_KINDS)}
+
+    def kernel_log(self):
+        """Structured array of every timed launch: kind, system, level, wgs, entries, ms."""
+        L = _lib.lib()
+        n = L.fcw_kernel_log(self._h, None, 0)
+        buf = np.zeros((max(n, 1), 6))
+        n = L.fcw_kernel_log(self._h, _lib.dptr(buf), n)
+        rec = np.zeros(n, dtype=[("kind", "U12"), ("system", "i4"), ("level", "i4"), ("wgs", "i8"),
+                                 ("entries", "f8"), ("ms", "f8")])
+        for i in range(n):
+            rec[i] = (self.KERNEL_KINDS[int(buf[i, 0])], int(buf[i, 1]), int(buf[i, 2]), int(buf[i, 3]), buf[i, 4],
+                      buf[i, 5])
+        return rec

@@ -128,7 +128,7 @@ class ServingEngine:
     """
 
     def __init__(self, hbm_budget_bytes=96 << 30, max_active=3, block=256, chunk_blocks=8, prep_threads=2,
-                 host_threads=8, precision="fp64", keep_outputs=True, prepare=None, streams=4):
+                 host_threads=8, precision="fp64", keep_outputs=True, prepare=None, streams=4, on_done=None):
         self.hbm_budget = int(hbm_budget_bytes)
         self.max_active = int(max_active)
         self.block = int(block)
@@ -142,6 +142,7 @@ class ServingEngine:
                                                            self.precision))
         self._admission = Admission(self.max_active, self.hbm_budget)
         self.trace = []
+        self.on_done = on_done  # called with each RequestResult as it completes
 
     def _admit(self, need):
         self._admission.admit(need)
@@ -178,6 +179,8 @@ class ServingEngine:
             if prep is not None:
                 prep.ctx.close()
             self._release(need)
+            if self.on_done is not None:
+                self.on_done(res)
 
     def run(self, requests, time_scale=1.0):
         """Replay requests at their arrival times (scaled); returns results in request order."""

@@ -147,11 +147,21 @@ int fcw_geometry_fields(fcw_ctx* ctx, const double* D, const int* src, int k, in
  * Returns the number of values written. */
 int fcw_stage_times(fcw_ctx* ctx, double* out, int len);
 
-/* Per-kernel timing with HIP events around every launch on the context's stream (0 = off).
- * fcw_kernel_times writes (total_ms, launches) for the forward and backward panel-product kernels
- * and the metric kernel: out[0..5]. */
+/* Per-kernel timing with HIP events around every launch on the context's streams (0 = off).
+ * fcw_kernel_times writes (total_ms, launches) per kernel kind, in the order of fcw_kernel_log's
+ * kinds (FP64 forward sweep, FP64 backward sweep, metric kernel, FP32 forward, FP32 backward,
+ * divergence, chunk statistics, residual): out[0..15]. */
 int fcw_set_profiling(fcw_ctx* ctx, int on);
 int fcw_kernel_times(fcw_ctx* ctx, double* out, int len);
+
+/* Per-launch record of the timed kernels since profiling was last switched on, 6 doubles per launch:
+ * kind (0 FP64 forward sweep, 1 FP64 backward sweep, 2 metric kernel, 3 FP32 forward sweep, 4 FP32
+ * backward sweep, 5 divergence, 6 chunk statistics, 7 FP64 residual), system (0 heat, 1 Poisson,
+ * -1 none), tree level (0 = leaves, -1 none), workgroups, Cholesky factor entries of the supernodes
+ * the launch covers (sum of s(s+1)/2 + s*b; sweeps only), milliseconds. With out = NULL returns the
+ * number of records held; else writes at most max_rows and returns how many. No reference
+ * counterpart (the reference times whole stages, fastcw_lean.py:527-530). */
+int fcw_kernel_log(fcw_ctx* ctx, double* out, int max_rows);
 
 /* Library self-description for the backend self-check; replaces the binary inspection of
  * backend_check.verify_suitesparse (backend_check.py:32-73). */
