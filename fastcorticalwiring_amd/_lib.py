@@ -75,6 +75,9 @@ class Info(ctypes.Structure):
         ("intrinsic_skipped", ctypes.c_int),
         ("mollify_delta", ctypes.c_double),
         ("split_levels", ctypes.c_int),
+        ("op_faces", ctypes.c_int),
+        ("tufted_cover", ctypes.c_int),
+        ("nonmanifold_edges", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -121,7 +124,7 @@ EXPORTED_SYMBOLS = (
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
     "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse",
-    "fcw_wiring_sources_noalloc", "fcw_kernel_log",
+    "fcw_wiring_sources_noalloc", "fcw_kernel_log", "fcw_host_intrinsic_cover",
 )
 
 _lib = None
@@ -171,6 +174,9 @@ def lib():
     L.fcw_host_factor_solve.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
     L.fcw_host_intrinsic.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, _ip, _dp, _dp, _dp, _dp]
+    L.fcw_host_intrinsic_cover.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp,
+                                           _dp, _dp, _dp]
+    L.fcw_host_intrinsic_cover.restype = ctypes.c_int
     L.fcw_host_analyse.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip, _ip,
                                    _dp, ctypes.POINTER(ctypes.c_uint64)]
     L.fcw_mesh_prep.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int,
@@ -265,6 +271,27 @@ def host_intrinsic(V, F, mollify=1e-6):
     check(lib().fcw_host_intrinsic(dptr(V), V.shape[0], iptr(F), m, float(mollify), iptr(faces), dptr(length),
                                    dptr(area), dptr(gram), dptr(stats)))
     return faces, length, area, gram, dict(flips=int(stats[0]), skipped=int(stats[1]), delta=float(stats[2]))
+
+
+def host_intrinsic_cover(V, F, mollify=1e-6):
+    """CPU-only: the intrinsic Delaunay triangulation for any mesh (tufted cover for non-manifold
+    edges). Returns (faces mo x 3, lengths mo x 3, weight mo, area mo (x weight), gram mo x 6, stats)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    m = F.shape[0]
+    faces = np.empty((2 * m, 3), np.int32)
+    length = np.empty((2 * m, 3))
+    weight = np.empty(2 * m)
+    area = np.empty(2 * m)
+    gram = np.empty((2 * m, 6))
+    stats = np.zeros(5)
+    mo = ctypes.c_int(0)
+    check(lib().fcw_host_intrinsic_cover(dptr(V), V.shape[0], iptr(F), m, float(mollify), ctypes.byref(mo),
+                                         iptr(faces), dptr(length), dptr(weight), dptr(area), dptr(gram), dptr(stats)))
+    k = mo.value
+    return (faces[:k], length[:k], weight[:k], area[:k], gram[:k],
+            dict(flips=int(stats[0]), skipped=int(stats[1]), delta=float(stats[2]), cover=bool(stats[3]),
+                 nonmanifold_edges=int(stats[4])))
 
 
 def mesh_prep_device(V, F, mask, start=0, device=0):

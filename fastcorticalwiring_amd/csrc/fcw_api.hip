@@ -184,7 +184,7 @@ struct fcw_ctx {
     fcw::Symbolic sym;
     fcw_options opt{};
     fcw_info info{};
-    int n = 0, m = 0, ldk = 0;
+    int n = 0, m = 0, mo = 0, ldk = 0;  // vertices, surface faces, operator faces, padded block width
     int64_t dev_bytes = 0;
     ScratchSlot* noalloc = nullptr;  // this device's scratch array (fcw_wiring_sources_noalloc)
     std::vector<std::unique_ptr<Workspace>> ws;
@@ -259,7 +259,7 @@ struct fcw_ctx {
         md.g11 = g11.p;
         md.g12 = g12.p;
         md.g22 = g22.p;
-        md.mo = m;
+        md.mo = mo;
         md.op_faces = robust ? op_faces.p : faces.p;
         md.op_area = robust ? op_area.p : area.p;
         md.op_gram = robust ? op_gram.p : gram.p;
@@ -905,12 +905,15 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         // (same vertices, flipped faces); the metric kernels keep the surface's own faces
         fcw::Intrinsic intr;
         const int* Fop = F;
+        int mop = m;  // faces of the operator triangulation
         C->robust = opt.robust != 0;
         if (C->robust) {
             if (fcw::intrinsic_delaunay(n, Vin, m, F, fcw::kMollify, intr, err)) throw Fail(err);
             Fop = intr.faces.data();
+            mop = (int)(intr.faces.size() / 3);
         }
-        if (fcw::analyse(n, Vin, m, Fop, opt.leaf_size > 0 ? opt.leaf_size : 64, S, err)) throw Fail(err);
+        C->mo = mop;
+        if (fcw::analyse(n, Vin, mop, Fop, opt.leaf_size > 0 ? opt.leaf_size : 64, S, err)) throw Fail(err);
         C->node_of_pos.assign(n, -1);
         for (int id = 0; id < S.nnodes; ++id)
             for (int p = S.node_start[id]; p < S.node_start[id] + S.node_s[id]; ++p) C->node_of_pos[p] = id;
@@ -963,11 +966,13 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         fcw::launch_face_geometry(st, m, C->V.p, C->faces.p, C->area.p, C->gram.p, C->larea.p, C->g11.p, C->g12.p,
                                   C->g22.p);
         if (C->robust) {
-            // operator faces in the analysis's order; corner c keeps its input corner's opposite edge
-            std::vector<double> len(3 * (size_t)m), oa(m), og(6 * (size_t)m);
-            for (int f = 0; f < m; ++f)
+            // operator faces in the analysis's order; corner c keeps its input corner's opposite edge.
+            // A face of the tufted cover left without its mirror copy counts half its area.
+            std::vector<double> len(3 * (size_t)mop), oa(mop), og(6 * (size_t)mop);
+            for (int f = 0; f < mop; ++f)
                 for (int c = 0; c < 3; ++c) len[3 * (size_t)f + c] = intr.len[3 * (size_t)S.face_perm[f] + c];
-            fcw::intrinsic_operators(m, len.data(), oa.data(), og.data());
+            fcw::intrinsic_operators(mop, len.data(), oa.data(), og.data());
+            for (int f = 0; f < mop; ++f) oa[f] *= intr.weight[S.face_perm[f]];
             C->op_faces.upload(S.faces.data(), S.faces.size(), tb);
             C->op_area.upload(oa.data(), oa.size(), tb);
             C->op_gram.upload(og.data(), og.size(), tb);
@@ -1130,6 +1135,9 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         I.intrinsic_flips = intr.flips;
         I.intrinsic_skipped = intr.skipped;
         I.mollify_delta = intr.delta;
+        I.op_faces = mop;
+        I.tufted_cover = intr.cover;
+        I.nonmanifold_edges = intr.nonmanifold_edges;
         I.mean_edge = mean_edge;
         I.setup_seconds[0] = t1 - t0;
         I.setup_seconds[1] = t2 - t1;
@@ -1415,6 +1423,9 @@ int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double molli
         fcw::Intrinsic I;
         std::string err;
         if (fcw::intrinsic_delaunay(n, V, m, F, mollify, I, err)) throw Fail(err);
+        if (I.cover)
+            throw Fail("non-manifold or inconsistently oriented edge: the triangulation is built on the tufted cover, "
+                       "use fcw_host_intrinsic_cover");
         std::copy(I.faces.begin(), I.faces.end(), faces_out);
         std::copy(I.len.begin(), I.len.end(), len_out);
         if (area_out && gram_out) fcw::intrinsic_operators(m, I.len.data(), area_out, gram_out);
@@ -1456,6 +1467,33 @@ void host_assemble(const fcw::Symbolic& S, const double* V, const int* F, std::v
 }  // namespace
 
 extern "C" {
+
+int fcw_host_intrinsic_cover(const double* V, int n, const int* F, int m, double mollify, int* mo, int* faces_out,
+                             double* len_out, double* weight_out, double* area_out, double* gram_out, double* stats) {
+    return guard([&] {
+        if (!V || !F || !mo || !faces_out || !len_out || !weight_out) throw Fail("null argument");
+        check_faces(n, V, m, F);
+        fcw::Intrinsic I;
+        std::string err;
+        if (fcw::intrinsic_delaunay(n, V, m, F, mollify, I, err)) throw Fail(err);
+        const int k = (int)(I.faces.size() / 3);
+        *mo = k;
+        std::copy(I.faces.begin(), I.faces.end(), faces_out);
+        std::copy(I.len.begin(), I.len.end(), len_out);
+        std::copy(I.weight.begin(), I.weight.end(), weight_out);
+        if (area_out && gram_out) {
+            fcw::intrinsic_operators(k, I.len.data(), area_out, gram_out);
+            for (int f = 0; f < k; ++f) area_out[f] *= I.weight[f];
+        }
+        if (stats) {
+            stats[0] = I.flips;
+            stats[1] = I.skipped;
+            stats[2] = I.delta;
+            stats[3] = I.cover;
+            stats[4] = I.nonmanifold_edges;
+        }
+    });
+}
 
 int fcw_host_analyse(const double* V, int n, const int* F, int m, int leaf_size, int par_depth, int* perm,
                      int* node_start, int* node_s, int* node_b, double* stats, uint64_t* panel_hash) {

@@ -62,12 +62,18 @@ int analyse(int n, const double* V, int m, const int* F, int leaf_size, Symbolic
 // Intrinsic Delaunay triangulation of the surface (V, F) after mollification with
 // eps = mollify * mean edge length (intrinsic.cpp): same vertices, flipped faces, and the length
 // of the edge opposite each corner.
+// A mesh with non-manifold edges (or neighbours of inconsistent orientation) is flipped on its
+// tufted cover, whose faces come back merged with their mirror copies: weight 1 for a merged pair,
+// 1/2 for a cover face left without its copy (the operators use area * weight).
 struct Intrinsic {
-    std::vector<int> faces;   // m x 3
-    std::vector<double> len;  // m x 3: edge opposite corner c
+    std::vector<int> faces;      // mo x 3 (mo = m for an edge-manifold mesh, at most 2m on the cover)
+    std::vector<double> len;     // mo x 3: edge opposite corner c
+    std::vector<double> weight;  // mo
     int flips = 0;
-    int skipped = 0;          // non-Delaunay edges whose flip would fold a face
-    double delta = 0.0;       // added to every edge length
+    int skipped = 0;             // non-Delaunay edges whose flip would fold a face
+    double delta = 0.0;          // added to every edge length
+    int cover = 0;               // 1: built on the tufted cover
+    int nonmanifold_edges = 0;   // input edges with three or more faces
 };
 constexpr double kMollify = 1e-6;
 int intrinsic_delaunay(int n, const double* V, int m, const int* F, double mollify, Intrinsic& out, std::string& err);

@@ -17,7 +17,18 @@
 //
 // Degenerate flips (both triangles of the edge having the same opposite vertex) are skipped and
 // counted; the cotan operators of the result need only the faces and their edge lengths.
+//
+// Non-manifold edges (three or more faces) and inconsistently oriented neighbours have no
+// halfedge twin, so such a mesh is first replaced by its tufted cover (Sharp & Crane 2020,
+// section 4.2): every face is taken twice, once per side, and around every edge the face sides
+// are glued in the cyclic order of the faces' angles about the edge (each wedge between two
+// consecutive faces glues the sides that face into it; a boundary edge glues a face's two sides
+// to each other). The cover is edge-manifold and consistently oriented, so the flips run on it
+// unchanged; its operators count every face twice, so each cover face carries half its area.
+// After flipping, a cover face and its mirror copy (same vertices, same lengths) are merged back
+// into one face of full area, which leaves a manifold mesh's result exactly as without the cover.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -45,54 +56,219 @@ inline double cot_opposite(double a, double b, double c) {
     return (a * a + b * b - c * c) / (4.0 * A);
 }
 
+inline size_t next_he(size_t h) { return h - h % 3 + (h % 3 + 1) % 3; }
+inline size_t prev_he(size_t h) { return h - h % 3 + (h % 3 + 2) % 3; }
+inline uint64_t edge_key(int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
+
+inline double dist3(const double* a, const double* b) {
+    return std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+// Halfedge twins of a mesh whose every edge has at most two faces, oriented against each other.
+// Returns false if some directed edge appears twice (a non-manifold edge, or two neighbours with
+// inconsistent orientation).
+bool manifold_twins(const std::vector<int>& T, std::vector<int64_t>& twin) {
+    const size_t nh = T.size();
+    twin.assign(nh, -1);
+    std::unordered_map<uint64_t, int64_t> dir;
+    dir.reserve(nh * 2);
+    for (size_t h = 0; h < nh; ++h)
+        if (!dir.emplace(edge_key(T[h], T[next_he(h)]), (int64_t)h).second) return false;
+    for (size_t h = 0; h < nh; ++h) {
+        auto it = dir.find(edge_key(T[next_he(h)], T[h]));
+        if (it != dir.end()) twin[h] = it->second;
+    }
+    return true;
+}
+
+// The tufted cover of (V, F): cover face 2f + s is face f seen from side s (s = 0: the vertex
+// order of f; s = 1: reversed, (a, c, b)), and the twins glue the sides around every edge in the
+// cyclic order of the faces about it. Returns the number of edges with three or more faces.
+int tufted_cover(const double* V, int m, const int* F, std::vector<int>& T, std::vector<int64_t>& twin,
+                 std::string& err) {
+    T.resize(6 * (size_t)m);
+    for (int f = 0; f < m; ++f) {
+        const int a = F[3 * (size_t)f], b = F[3 * (size_t)f + 1], c = F[3 * (size_t)f + 2];
+        int* t0 = &T[6 * (size_t)f];
+        t0[0] = a, t0[1] = b, t0[2] = c;  // side 0
+        t0[3] = a, t0[4] = c, t0[5] = b;  // side 1
+    }
+    twin.assign(T.size(), -1);
+    // cover halfedge of face f, side s, on the input halfedge 3f + c (from F[3f + c] to F[3f + c + 1]):
+    // side 0 keeps c, side 1 runs the edge the other way in slot 2 - c
+    auto cover_he = [](int f, int side, int c) { return (int64_t)(6 * (size_t)f + 3 * side + (side ? 2 - c : c)); };
+    std::unordered_map<uint64_t, std::vector<int>> around;  // undirected edge -> input halfedges
+    around.reserve(3 * (size_t)m);
+    for (size_t h = 0; h < 3 * (size_t)m; ++h) {
+        int i = F[h], j = F[next_he(h)];
+        if (i > j) std::swap(i, j);
+        around[edge_key(i, j)].push_back((int)h);
+    }
+    int nonmanifold = 0;
+    std::vector<std::pair<double, int>> ring;
+    for (auto& kv : around) {
+        const int i = (int)(kv.first >> 32), j = (int)(kv.first & 0xffffffffu);
+        const std::vector<int>& H = kv.second;
+        if (H.size() >= 3) ++nonmanifold;
+        const double* xi = V + 3 * (size_t)i;
+        const double* xj = V + 3 * (size_t)j;
+        double e[3] = {xj[0] - xi[0], xj[1] - xi[1], xj[2] - xi[2]};
+        const double el = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        for (double& x : e) x /= el;
+        // per incident face: its opposite vertex's direction about the edge, and which side of the
+        // face looks towards increasing angle
+        ring.clear();
+        std::vector<char> plus_up(H.size());
+        double u[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+        for (size_t q = 0; q < H.size(); ++q) {
+            const int h = H[q];
+            const int f = h / 3;
+            const int k = F[3 * (size_t)f + (h % 3 + 2) % 3];
+            const double* xk = V + 3 * (size_t)k;
+            double w[3] = {xk[0] - xi[0], xk[1] - xi[1], xk[2] - xi[2]};
+            const double we = w[0] * e[0] + w[1] * e[1] + w[2] * e[2];
+            for (int d = 0; d < 3; ++d) w[d] -= we * e[d];
+            if (q == 0) {
+                const double wl = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                for (int d = 0; d < 3; ++d) u[d] = w[d] / wl;
+                v[0] = e[1] * u[2] - e[2] * u[1];
+                v[1] = e[2] * u[0] - e[0] * u[2];
+                v[2] = e[0] * u[1] - e[1] * u[0];
+            }
+            const double theta = std::atan2(w[0] * v[0] + w[1] * v[1] + w[2] * v[2], w[0] * u[0] + w[1] * u[1] + w[2] * u[2]);
+            ring.push_back({theta, (int)q});
+            // direction of increasing angle at the face: e x w; the face's normal from its vertex order
+            const double t[3] = {e[1] * w[2] - e[2] * w[1], e[2] * w[0] - e[0] * w[2], e[0] * w[1] - e[1] * w[0]};
+            const double* p0 = V + 3 * (size_t)F[3 * (size_t)f];
+            const double* p1 = V + 3 * (size_t)F[3 * (size_t)f + 1];
+            const double* p2 = V + 3 * (size_t)F[3 * (size_t)f + 2];
+            const double a1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+            const double a2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+            const double nrm[3] = {a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2], a1[0] * a2[1] - a1[1] * a2[0]};
+            plus_up[q] = (nrm[0] * t[0] + nrm[1] * t[1] + nrm[2] * t[2]) > 0.0;
+        }
+        std::sort(ring.begin(), ring.end(), [&](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+            return x.first != y.first ? x.first < y.first : H[x.second] < H[y.second];
+        });
+        // the wedge after face a (in increasing angle) glues a's side facing up with b's side
+        // facing down; a boundary edge's one face glues its two sides
+        for (size_t r = 0; r < ring.size(); ++r) {
+            const int qa = ring[r].second, qb = ring[(r + 1) % ring.size()].second;
+            const int ha = H[qa], hb = H[qb];
+            const int64_t ca = cover_he(ha / 3, plus_up[qa] ? 0 : 1, ha % 3);
+            const int64_t cb = cover_he(hb / 3, plus_up[qb] ? 1 : 0, hb % 3);
+            if (T[(size_t)ca] != T[next_he((size_t)cb)] || T[(size_t)cb] != T[next_he((size_t)ca)] ||
+                twin[(size_t)ca] >= 0 || twin[(size_t)cb] >= 0) {
+                err = "tufted cover: inconsistent gluing";
+                return -1;
+            }
+            twin[(size_t)ca] = cb;
+            twin[(size_t)cb] = ca;
+        }
+    }
+    return nonmanifold;
+}
+
+// Merge each cover face with its mirror copy (same vertices, same lengths to 1e-12): one face of
+// weight 1; a face left without its copy keeps weight 1/2.
+void merge_mirrors(Intrinsic& out) {
+    const size_t mf = out.faces.size() / 3;
+    std::vector<int> key_order(mf);
+    for (size_t f = 0; f < mf; ++f) key_order[f] = (int)f;
+    auto sorted3 = [&](size_t f) {
+        std::array<int, 3> v = {out.faces[3 * f], out.faces[3 * f + 1], out.faces[3 * f + 2]};
+        std::sort(v.begin(), v.end());
+        return v;
+    };
+    auto lens = [&](size_t f) {
+        std::array<double, 3> l = {out.len[3 * f], out.len[3 * f + 1], out.len[3 * f + 2]};
+        std::sort(l.begin(), l.end());
+        return l;
+    };
+    std::stable_sort(key_order.begin(), key_order.end(), [&](int a, int b) { return sorted3(a) < sorted3(b); });
+    std::vector<char> used(mf, 0);
+    std::vector<int> faces;
+    std::vector<double> len, weight;
+    auto emit = [&](size_t f, double wgt) {
+        for (int c = 0; c < 3; ++c) {
+            faces.push_back(out.faces[3 * f + c]);
+            len.push_back(out.len[3 * f + c]);
+        }
+        weight.push_back(wgt);
+    };
+    for (size_t g0 = 0; g0 < mf;) {
+        size_t g1 = g0;
+        const auto k0 = sorted3(key_order[g0]);
+        while (g1 < mf && sorted3(key_order[g1]) == k0) ++g1;
+        for (size_t a = g0; a < g1; ++a) {
+            const size_t fa = key_order[a];
+            if (used[fa]) continue;
+            used[fa] = 1;
+            const auto la = lens(fa);
+            size_t match = mf;
+            for (size_t b = a + 1; b < g1 && match == mf; ++b) {
+                const size_t fb = key_order[b];
+                if (used[fb]) continue;
+                const auto lb = lens(fb);
+                bool same = true;
+                for (int c = 0; c < 3; ++c) same = same && std::fabs(la[c] - lb[c]) <= 1e-12 * la[c];
+                if (same) match = fb;
+            }
+            if (match != mf) used[match] = 1;
+            emit(fa, match != mf ? 1.0 : 0.5);
+        }
+        g0 = g1;
+    }
+    out.faces.swap(faces);
+    out.len.swap(len);
+    out.weight.swap(weight);
+}
+
 }  // namespace
 
 int intrinsic_delaunay(int n, const double* V, int m, const int* F, double mollify, Intrinsic& out, std::string& err) {
     out = Intrinsic();
-    out.faces.assign(F, F + 3 * (size_t)m);
-    std::vector<int>& T = out.faces;
-    const size_t nh = 3 * (size_t)m;
+    for (size_t h = 0; h < 3 * (size_t)m; ++h)
+        if (F[h] == F[next_he(h)]) {
+            err = "face with a repeated vertex";
+            return 1;
+        }
     // halfedge h = 3f + c runs from T[h] to T[3f + (c + 1) % 3]
-    auto next = [](size_t h) { return h - h % 3 + (h % 3 + 1) % 3; };
-    auto prev = [](size_t h) { return h - h % 3 + (h % 3 + 2) % 3; };
-    std::vector<int64_t> twin(nh, -1);
-    std::vector<double> len(nh);
-    {
-        std::unordered_map<uint64_t, int64_t> dir;
-        dir.reserve(nh * 2);
-        auto key = [](int a, int b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; };
-        for (size_t h = 0; h < nh; ++h) {
-            const int a = T[h], b = T[next(h)];
-            if (a == b) {
-                err = "face with a repeated vertex";
-                return 1;
-            }
-            if (!dir.emplace(key(a, b), (int64_t)h).second) {
-                err = "non-manifold or inconsistently oriented edge";
-                return 1;
-            }
-        }
-        for (size_t h = 0; h < nh; ++h) {
-            auto it = dir.find(key(T[next(h)], T[h]));
-            if (it != dir.end()) twin[h] = it->second;
-        }
+    out.faces.assign(F, F + 3 * (size_t)m);
+    std::vector<int64_t> twin;
+    if (!manifold_twins(out.faces, twin)) {
+        const int nm = tufted_cover(V, m, F, out.faces, twin, err);
+        if (nm < 0) return 1;
+        out.cover = 1;
+        out.nonmanifold_edges = nm;
     }
+    std::vector<int>& T = out.faces;
+    const size_t nh = T.size();
+    const int mf = (int)(nh / 3);
+    auto next = next_he;
+    auto prev = prev_he;
+    std::vector<double> len(nh);
+    for (size_t h = 0; h < nh; ++h) len[h] = dist3(V + 3 * (size_t)T[h], V + 3 * (size_t)T[next(h)]);
+    // mean length over the input's distinct edges
     double sum = 0.0;
     int64_t ne = 0;
-    for (size_t h = 0; h < nh; ++h) {
-        const double* a = V + 3 * (size_t)T[h];
-        const double* b = V + 3 * (size_t)T[next(h)];
-        len[h] = std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
-        if (twin[h] < 0 || (int64_t)h < twin[h]) {
-            sum += len[h];
-            ++ne;
+    {
+        std::vector<uint64_t> e;
+        e.reserve(3 * (size_t)m);
+        for (size_t h = 0; h < 3 * (size_t)m; ++h) {
+            const int a = std::min(F[h], F[next(h)]), b = std::max(F[h], F[next(h)]);
+            e.push_back(edge_key(a, b));
         }
+        std::sort(e.begin(), e.end());
+        e.erase(std::unique(e.begin(), e.end()), e.end());
+        for (uint64_t k : e) sum += dist3(V + 3 * (size_t)(k >> 32), V + 3 * (size_t)(k & 0xffffffffu));
+        ne = (int64_t)e.size();
     }
     (void)n;
     // 1. mollification
     const double eps = mollify * (ne ? sum / ne : 0.0);
     double delta = 0.0;
-    for (int f = 0; f < m; ++f) {
+    for (int f = 0; f < mf; ++f) {
         const double l0 = len[3 * (size_t)f], l1 = len[3 * (size_t)f + 1], l2 = len[3 * (size_t)f + 2];
         delta = std::max({delta, eps - (l0 + l1 - l2), eps - (l1 + l2 - l0), eps - (l2 + l0 - l1)});
     }
@@ -154,8 +330,12 @@ int intrinsic_delaunay(int n, const double* V, int m, const int* F, double molli
     }
     // length of the edge opposite corner c = halfedge (c + 1) % 3 of the face
     out.len.resize(nh);
-    for (int f = 0; f < m; ++f)
+    for (int f = 0; f < mf; ++f)
         for (int c = 0; c < 3; ++c) out.len[3 * (size_t)f + c] = len[3 * (size_t)f + (c + 1) % 3];
+    if (out.cover)
+        merge_mirrors(out);
+    else
+        out.weight.assign(mf, 1.0);
     return 0;
 }
 

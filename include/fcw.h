@@ -82,6 +82,9 @@ typedef struct {
     int intrinsic_skipped;      /* non-Delaunay edges left because flipping them would fold a face */
     double mollify_delta;       /* length added to every edge by the mollification (robust) */
     int split_levels;           /* sweep launches per block whose depth is split (0: none) */
+    int op_faces;               /* faces of the operator triangulation (robust: intrinsic, merged cover) */
+    int tufted_cover;           /* robust: 1 if the operators were built on the tufted cover */
+    int nonmanifold_edges;      /* robust: input edges with three or more faces */
 } fcw_info;
 
 typedef struct {
@@ -179,6 +182,17 @@ int fcw_calibration_copy(int64_t bytes, int device);
 int fcw_host_factor_solve(const double* V, int n, const int* F, int m, double l_scale, double m_scale,
                           int leaf_size, const double* B, int k, double* X, double* stats);
 
+/* Host-only: as fcw_host_intrinsic for any mesh, including non-manifold edges and neighbours of
+ * inconsistent orientation, which potpourri3d's robust mode accepts (distance_engines.py:72-133;
+ * core_analysis.py:609-636 lets such meshes through). Those meshes are flipped on their tufted cover
+ * (each face once per side, glued around every edge in the cyclic order of its faces), whose faces
+ * come back merged with their mirror copies. Buffers hold up to 2m faces: *mo = faces written,
+ * faces_out mo x 3, len_out mo x 3, weight_out mo (1, or 1/2 for a cover face without its copy),
+ * area_out mo (area x weight, may be NULL), gram_out mo x 6 (may be NULL). stats[0..4] = flips,
+ * skipped flips, mollification delta, 1 if the cover was used, edges with three or more faces. */
+int fcw_host_intrinsic_cover(const double* V, int n, const int* F, int m, double mollify, int* mo, int* faces_out,
+                             double* len_out, double* weight_out, double* area_out, double* gram_out, double* stats);
+
 /* Host-only view of the analysis (no GPU): the nested-dissection ordering and supernodes that
  * fcw_create builds, with the dissection's top `par_depth` levels split over threads (< 0: the
  * default rule, 2 on meshes of 20,000 vertices or more, else 0). The result does not depend on
@@ -221,7 +235,8 @@ int fcw_mesh_prep(const double* V, int n, const int* F, int m, const unsigned ch
  * triangulation inside potpourri3d's robust mode, distance_engines.py:119-125, which has no
  * callable counterpart). faces_out m x 3 (same vertices, flipped connectivity), len_out m x 3
  * (length of the edge opposite each corner), area_out m, gram_out m x 6 (grad B_a . grad B_b:
- * G00 G01 G02 G11 G12 G22). stats[0..2] = flips, skipped flips, mollification delta. */
+ * G00 G01 G02 G11 G12 G22). stats[0..2] = flips, skipped flips, mollification delta. Edge-manifold,
+ * consistently oriented meshes only (else an error naming fcw_host_intrinsic_cover). */
 int fcw_host_intrinsic(const double* V, int n, const int* F, int m, double mollify, int* faces_out, double* len_out,
                        double* area_out, double* gram_out, double* stats);
 

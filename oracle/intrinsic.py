@@ -14,10 +14,16 @@ version); its robust mode follows the published algorithm restated below:
   length taken from the planar layout of the pair, until none is left;
 * the cotan Laplacian L_ij = -(cot a_ij + cot b_ij)/2, lumped mass sum(A_f)/3, the heat method
   on that triangulation (t = t_coef * mean edge length of the input surface), the divergence of
-  the normalised gradient computed in each triangle's own planar layout.
+  the normalised gradient computed in each triangle's own planar layout;
+* for meshes with non-manifold edges or inconsistently oriented neighbours, the tufted cover
+  (Sharp & Crane 2020, section 4.2): each face taken once per side, the sides glued around every
+  edge in the cyclic order of the faces about it, the flips run on that cover, and every cover
+  face weighted 1/2 (each input face is counted twice); a face and its mirror copy that end up
+  identical are merged into one face of weight 1.
 
 This restatement shares no code with the library's (intrinsic.cpp uses halfedge arrays and
-the Gram form; this uses a directed-edge map, cotangents and planar layouts).
+the Gram form; this uses a directed-edge map, or for the cover a (face, slot) twin map,
+cotangents and planar layouts).
 Parity is pinned by invariants (every interior edge Delaunay, area preserved, no flips and the
 extrinsic operators back on a Delaunay input, a planar known answer) in tests/test_oracle.py;
 against potpourri3d itself it is unpinned (not installed).
@@ -113,6 +119,164 @@ def intrinsic_delaunay(V, F, mollify=1e-6, max_sweeps=1000):
     return Fo, Lo, flips, delta
 
 
+def _is_edge_manifold(F):
+    seen = set()
+    for f in np.asarray(F).tolist():
+        for c in range(3):
+            d = (f[c], f[(c + 1) % 3])
+            if d in seen:
+                return False
+            seen.add(d)
+    return True
+
+
+def tufted_cover(V, F):
+    """Cover faces (face f side 0 = f's vertex order, side 1 = reversed as (c, b, a); cover face
+    index 2f + side) and the twin map {(cover face, slot): (cover face, slot)}, slot c being the
+    edge from vertex c to vertex c + 1, glued around every input edge in the faces' angular order."""
+    V = np.asarray(V, dtype=np.float64)
+    F = [list(map(int, f)) for f in np.asarray(F)]
+    faces = []
+    for a, b, c in F:
+        faces.append([a, b, c])
+        faces.append([c, b, a])
+    about = {}
+    for fi, f in enumerate(F):
+        for c in range(3):
+            a, b = f[c], f[(c + 1) % 3]
+            about.setdefault((min(a, b), max(a, b)), []).append(fi)
+
+    def slot(cf, a, b):
+        t = faces[cf]
+        for c in range(3):
+            if t[c] == a and t[(c + 1) % 3] == b:
+                return c
+        return None
+
+    twin = {}
+    nonmanifold = 0
+    for (i, j), fl in about.items():
+        nonmanifold += len(fl) >= 3
+        e = V[j] - V[i]
+        e = e / np.linalg.norm(e)
+        ws, ups = [], []
+        for fi in fl:
+            k = [v for v in F[fi] if v != i and v != j][0]
+            w = V[k] - V[i]
+            w = w - np.dot(w, e) * e
+            ws.append(w)
+            p0, p1, p2 = V[F[fi][0]], V[F[fi][1]], V[F[fi][2]]
+            ups.append(float(np.dot(np.cross(p1 - p0, p2 - p0), np.cross(e, w))) > 0.0)
+        u = ws[0] / np.linalg.norm(ws[0])
+        v = np.cross(e, u)
+        theta = [math.atan2(float(np.dot(w, v)), float(np.dot(w, u))) for w in ws]
+        ring = sorted(range(len(fl)), key=lambda q: (theta[q], fl[q]))
+        for r in range(len(ring)):
+            qa, qb = ring[r], ring[(r + 1) % len(ring)]
+            ca = 2 * fl[qa] + (0 if ups[qa] else 1)  # side of face a facing increasing angle
+            cb = 2 * fl[qb] + (1 if ups[qb] else 0)  # side of face b facing decreasing angle
+            sa = slot(ca, i, j)
+            x, y = (i, j) if sa is not None else (j, i)
+            if sa is None:
+                sa = slot(ca, j, i)
+            sb = slot(cb, y, x)
+            if sb is None or (ca, sa) in twin or (cb, sb) in twin:
+                raise RuntimeError("tufted cover: inconsistent gluing")
+            twin[(ca, sa)] = (cb, sb)
+            twin[(cb, sb)] = (ca, sa)
+    return faces, twin, nonmanifold
+
+
+def tufted_delaunay(V, F, mollify=1e-6, max_sweeps=1000):
+    """Intrinsic Delaunay triangulation of the tufted cover: (faces, lengths opposite each corner,
+    weight per face, flips, delta, non-manifold edge count)."""
+    V = np.asarray(V, dtype=np.float64)
+    faces, twin, nonmanifold = tufted_cover(V, F)
+    Fl = [list(map(int, f)) for f in np.asarray(F)]
+    length = {}
+    for f in Fl:
+        for c in range(3):
+            a, b = f[c], f[(c + 1) % 3]
+            length.setdefault((min(a, b), max(a, b)), float(np.linalg.norm(V[a] - V[b])))
+    eps = mollify * (sum(length.values()) / len(length))
+    delta = 0.0
+    for f in Fl:
+        l = [length[(min(f[c], f[(c + 1) % 3]), max(f[c], f[(c + 1) % 3]))] for c in range(3)]
+        for c in range(3):
+            delta = max(delta, eps - (l[c] + l[(c + 1) % 3] - l[(c + 2) % 3]))
+    # slot lengths per cover face
+    flen = [[length[(min(t[c], t[(c + 1) % 3]), max(t[c], t[(c + 1) % 3]))] + delta for c in range(3)] for t in faces]
+    flips = 0
+    for _ in range(max_sweeps):
+        changed = False
+        for fi in range(len(faces)):
+            for c in range(3):
+                if (fi, c) not in twin:
+                    continue
+                gi, d = twin[(fi, c)]
+                if (gi, d) < (fi, c):
+                    continue
+                f, g = faces[fi], faces[gi]
+                i, j, k = f[c], f[(c + 1) % 3], f[(c + 2) % 3]
+                l = g[(d + 2) % 3]
+                if k == l:
+                    continue
+                cij, cjk, cki = flen[fi][c], flen[fi][(c + 1) % 3], flen[fi][(c + 2) % 3]
+                cil, clj = flen[gi][(d + 1) % 3], flen[gi][(d + 2) % 3]
+                if _cot(cjk, cki, cij) + _cot(cil, clj, cij) >= -1e-12:
+                    continue
+                xk = (cij * cij + cki * cki - cjk * cjk) / (2 * cij)
+                yk = math.sqrt(max(cki * cki - xk * xk, 0.0))
+                xl = (cij * cij + cil * cil - clj * clj) / (2 * cij)
+                yl = -math.sqrt(max(cil * cil - xl * xl, 0.0))
+                lkl = math.hypot(xk - xl, yk - yl)
+                out_ki = twin.get((fi, (c + 2) % 3))
+                out_jk = twin.get((fi, (c + 1) % 3))
+                out_il = twin.get((gi, (d + 1) % 3))
+                out_lj = twin.get((gi, (d + 2) % 3))
+                for key in ((fi, 0), (fi, 1), (fi, 2), (gi, 0), (gi, 1), (gi, 2)):
+                    twin.pop(key, None)
+                faces[fi] = [k, i, l]  # slots k->i, i->l, l->k
+                flen[fi] = [cki, cil, lkl]
+                faces[gi] = [l, j, k]  # slots l->j, j->k, k->l
+                flen[gi] = [clj, cjk, lkl]
+                pairs = [((fi, 0), out_ki), ((fi, 1), out_il), ((gi, 0), out_lj), ((gi, 1), out_jk),
+                         ((fi, 2), (gi, 2))]
+                for a, b in pairs:
+                    if b is not None:
+                        twin[a] = b
+                        twin[b] = a
+                flips += 1
+                changed = True
+        if not changed:
+            break
+    else:
+        raise RuntimeError("intrinsic Delaunay flips did not converge")
+    # merge a face with an identical mirror copy
+    groups = {}
+    for fi, t in enumerate(faces):
+        groups.setdefault(tuple(sorted(t)), []).append(fi)
+    Fo, Lo, W = [], [], []
+    for key in sorted(groups):
+        left = list(groups[key])
+        while left:
+            fa = left.pop(0)
+            la = sorted(flen[fa])
+            mate = None
+            for fb in left:
+                lb = sorted(flen[fb])
+                if all(abs(x - y) <= 1e-12 * x for x, y in zip(la, lb)):
+                    mate = fb
+                    break
+            if mate is not None:
+                left.remove(mate)
+            Fo.append(faces[fa])
+            fl = flen[fa]
+            Lo.append([fl[1], fl[2], fl[0]])  # opposite corner c = slot c + 1
+            W.append(1.0 if mate is not None else 0.5)
+    return np.array(Fo, dtype=np.int64), np.array(Lo), np.array(W), flips, delta, nonmanifold
+
+
 def is_delaunay(F, Lopp, tol=1e-10):
     """Every interior edge has cot alpha + cot beta >= -tol."""
     F = np.asarray(F)
@@ -126,9 +290,11 @@ def is_delaunay(F, Lopp, tol=1e-10):
     return all(sum(v) >= -tol for v in cots.values() if len(v) == 2)
 
 
-def intrinsic_operators(F, Lopp, n):
-    """(L, M, area): cotan Laplacian and lumped mass of the triangulation with the given lengths."""
+def intrinsic_operators(F, Lopp, n, weight=None):
+    """(L, M, area): cotan Laplacian and lumped mass of the triangulation with the given lengths;
+    each face's contribution times its weight (tufted cover), area returned unweighted."""
     F = np.asarray(F, dtype=np.int64)
+    wf = np.ones(F.shape[0]) if weight is None else np.asarray(weight, dtype=np.float64)
     rows, cols, vals = [], [], []
     mass = np.zeros(n)
     area = np.empty(F.shape[0])
@@ -140,22 +306,23 @@ def intrinsic_operators(F, Lopp, n):
         area[fi] = A
         for c in range(3):
             a, b = int(f[(c + 1) % 3]), int(f[(c + 2) % 3])
-            w = 0.5 * _cot(Lopp[fi][(c + 1) % 3], Lopp[fi][(c + 2) % 3], Lopp[fi][c])  # angle at corner c
+            w = 0.5 * wf[fi] * _cot(Lopp[fi][(c + 1) % 3], Lopp[fi][(c + 2) % 3], Lopp[fi][c])  # corner c
             rows += [a, b, a, b]
             cols += [b, a, a, b]
             vals += [-w, -w, w, w]
-            mass[int(f[c])] += A / 3.0
+            mass[int(f[c])] += wf[fi] * A / 3.0
     L = sp.csc_matrix((vals, (rows, cols)), shape=(n, n))
     L.sum_duplicates()
     return L, sp.diags(mass, format="csc"), area
 
 
-def layout_divergence(u, F, Lopp):
-    """sum_f A_f (X_f . grad B_i), X_f = -grad u/|grad u|, each face laid out in the plane."""
+def layout_divergence(u, F, Lopp, weight=None):
+    """sum_f w_f A_f (X_f . grad B_i), X_f = -grad u/|grad u|, each face laid out in the plane."""
     u = np.asarray(u, dtype=np.float64)
     if u.ndim == 1:
         u = u[:, None]
     F = np.asarray(F, dtype=np.int64)
+    wf = np.ones(F.shape[0]) if weight is None else np.asarray(weight, dtype=np.float64)
     out = np.zeros_like(u)
     for fi, f in enumerate(F):
         l0, l1, l2 = Lopp[fi]
@@ -173,18 +340,23 @@ def layout_divergence(u, F, Lopp):
         nn = np.sqrt(np.sum(grad * grad, axis=0))
         with np.errstate(divide="ignore", invalid="ignore"):
             X = np.where(nn > 1e-300, -grad / nn, 0.0)
-        out[f] += A * (g @ X)
+        out[f] += wf[fi] * A * (g @ X)
     return out
 
 
 class RobustHeatOracle:
     """FP64 heat-method distances on the intrinsic Delaunay triangulation (k sources per pass)."""
 
-    def __init__(self, vertices, faces, t_coef=1.0, poisson_shift=1e-10, t=None, mollify=1e-6):
+    def __init__(self, vertices, faces, t_coef=1.0, poisson_shift=1e-10, t=None, mollify=1e-6, cover=None):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
-        self.F, self.Lopp, self.flips, self.delta = intrinsic_delaunay(V, faces, mollify)
+        self.cover = (not _is_edge_manifold(faces)) if cover is None else bool(cover)
+        if self.cover:
+            self.F, self.Lopp, self.weight, self.flips, self.delta, _ = tufted_delaunay(V, faces, mollify)
+        else:
+            self.F, self.Lopp, self.flips, self.delta = intrinsic_delaunay(V, faces, mollify)
+            self.weight = np.ones(self.F.shape[0])
         n = V.shape[0]
-        self.L, self.M, self.area = intrinsic_operators(self.F, self.Lopp, n)
+        self.L, self.M, self.area = intrinsic_operators(self.F, self.Lopp, n, self.weight)
         self.mean_edge = mean_edge_length(V, faces)
         self.t = float(t) if t is not None else float(t_coef) * self.mean_edge ** 2
         sym = dict(diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
@@ -197,7 +369,7 @@ class RobustHeatOracle:
         rhs = np.zeros((self.n, src.size))
         rhs[src, np.arange(src.size)] = 1.0
         u = np.asarray(self._heat.solve(np.asfortranarray(rhs)))
-        div = layout_divergence(u, self.F, self.Lopp)
+        div = layout_divergence(u, self.F, self.Lopp, self.weight)
         phi = np.asarray(self._poisson.solve(np.asfortranarray(div)))
         phi = phi - phi[src, np.arange(src.size)][None, :]
         return np.ascontiguousarray(phi.T)

@@ -88,3 +88,37 @@ def test_robust_wiring_uses_the_surface_geometry(ell):
         assert e < 1e-7 and p < 1e-7, (i, e, p)
     e = np.max(rel_err(sc[ok, 5], ref["msd_weighted"][orig[ok]].astype(np.float64)))
     assert e < 1e-6  # the reference stores MSD as float32
+
+
+@pytest.mark.parametrize("height,fins", [(5.0, 1), (0.05, 1), (3.0, 3)])
+def test_robust_distances_on_non_manifold_meshes(height, fins):
+    """Meshes with an edge of 3-5 faces (fins hinged on one edge of a capped ellipsoid): the robust
+    operators are built on the tufted cover, and the device distances match the robust oracle's at
+    1e-9; the potpourri-compatible engine (robust always) accepts the mesh."""
+    import test_intrinsic as ti
+
+    from fastcorticalwiring_amd import distance_engines as de
+    from fastcorticalwiring_amd.engine import HeatContext
+    from oracle import analysis as oa
+    from oracle import intrinsic as oi
+
+    V, F = ti.fin_mesh(level=4, height=height, fins=fins)
+    mask = V[:, 2] < 0.8 * V[:, 2].max()
+    mesh = oa.OracleMesh(V, F, mask)
+    print(f"fin mesh: {mesh.n} vertices, height {height}, {fins} fin(s)", flush=True)
+    R = oi.RobustHeatOracle(mesh.V, mesh.F)
+    assert R.cover
+    src = np.concatenate([np.arange(0, mesh.n, 29)[:60], [mesh.n - 1]])  # the last fin apex too
+    with HeatContext(mesh.V, mesh.F, block=128, robust=True) as ctx:
+        assert ctx.info["tufted_cover"] == 1 and ctx.info["nonmanifold_edges"] == 1
+        assert ctx.info["op_faces"] == R.F.shape[0]
+        D = ctx.distances(src)
+    Dref = R.compute_distance_batch(src)
+    e = np.max(np.abs(D - Dref)) / np.max(np.abs(Dref))
+    print(f"  distances vs robust oracle: max abs / max {e:.2e}", flush=True)
+    assert e < 1e-9
+    eng = de.create_distance_engine("mi355x_potpourri", mesh.V, mesh.F, {"block": 64})
+    d = eng.compute_distance(int(src[3]))
+    Rp = oi.RobustHeatOracle(mesh.V, mesh.F, t=0.7 ** 2)
+    ref = Rp.compute_distance_batch([int(src[3])])[0]
+    assert np.max(np.abs(d - ref)) < 1e-9 * np.max(ref)

@@ -129,3 +129,102 @@ def test_robust_oracle_distances_close_to_great_circles():
     err_plain = np.mean(np.abs(d_plain - exact)) / R
     err_rob = np.mean(np.abs(d_rob - exact)) / R
     assert err_rob <= err_plain * 1.05 and err_rob < 0.05
+
+
+# ---- tufted cover (non-manifold edges, inconsistent orientation) --------------------------------
+
+def fin_mesh(level=3, height=5.0, fins=1, stretch=3.0):
+    """A stretched icosphere (radius 30 mm) with `fins` extra triangles hinged on the edge of its
+    face 0: that edge then has 2 + fins faces. A fin's apex sits `height` times the edge length
+    from the edge's midpoint, along the face's normal rotated by a different angle per fin."""
+    V, F = meshes.icosphere(level)
+    V = V * 30.0 * np.array([1.0, 1.0, stretch])
+    F = F.astype(np.int32)
+    i, j, k = (int(x) for x in F[0])
+    e = V[j] - V[i]
+    L = np.linalg.norm(e)
+    nrm = np.cross(e, V[k] - V[i])
+    nrm /= np.linalg.norm(nrm)
+    side = np.cross(nrm, e / L)
+    extra_v, extra_f = [], []
+    for q in range(fins):
+        ang = 0.35 + 0.5 * q
+        d = np.cos(ang) * nrm + np.sin(ang) * side
+        extra_v.append(0.5 * (V[i] + V[j]) + height * L * d)
+        extra_f.append([i, j, V.shape[0] + q])
+    return (np.ascontiguousarray(np.vstack([V] + extra_v)), np.ascontiguousarray(np.vstack([F, extra_f]), np.int32))
+
+
+def _surface_area(V, F):
+    return 0.5 * np.linalg.norm(np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]]), axis=1).sum()
+
+
+def _check_laplacian(L, n):
+    """Symmetric, rows summing to zero, PSD, and no positive off-diagonal weight (every edge of
+    the cover Delaunay)."""
+    scale = abs(L).max()
+    assert abs(L - L.T).max() <= 1e-14 * scale
+    assert np.max(np.abs(L @ np.ones(n))) <= 1e-12 * scale
+    assert np.linalg.eigvalsh(L.toarray()).min() >= -1e-12 * scale
+    off = (L - sp_diag(L)).tocoo()
+    assert off.data.max() <= 1e-12 * scale
+
+
+def test_cover_of_a_manifold_mesh_is_the_plain_triangulation():
+    """On an edge-manifold, consistently oriented mesh the tufted cover is two copies of the surface
+    that flip alike, so merging the mirror copies gives back the plain intrinsic triangulation."""
+    V, F = _ellipsoid()
+    n = V.shape[0]
+    Fo, Lo, flips, _ = oi.intrinsic_delaunay(V, F)
+    Fc, Lc, Wc, flips_c, _, nm = oi.tufted_delaunay(V, F)
+    assert nm == 0 and flips_c == 2 * flips and np.all(Wc == 1.0) and Fc.shape == Fo.shape
+    L0, M0, _ = oi.intrinsic_operators(Fo, Lo, n)
+    L1, M1, _ = oi.intrinsic_operators(Fc, Lc, n, Wc)
+    assert abs(L1 - L0).max() <= 1e-12 * abs(L0).max()
+    assert np.allclose(M1.diagonal(), M0.diagonal(), rtol=1e-14)
+    faces, lens, w, area, gram, st = _lib.host_intrinsic_cover(V, F)
+    assert not st["cover"] and np.all(w == 1.0)
+
+
+def test_orientation_does_not_change_the_operators():
+    """Reversing every other face makes neighbours disagree on orientation (no halfedge twins); the
+    cover then gives the same Laplacian as the consistently oriented surface."""
+    V, F = _ellipsoid()
+    n = V.shape[0]
+    F2 = F.copy()
+    F2[::2] = F2[::2][:, [0, 2, 1]]
+    with pytest.raises(_lib.FcwError, match="fcw_host_intrinsic_cover"):
+        _lib.host_intrinsic(V, F2)
+    f0, l0, a0, g0, _ = _lib.host_intrinsic(V, F)
+    f2, l2, w2, a2, g2, st = _lib.host_intrinsic_cover(V, F2)
+    assert st["cover"] and st["nonmanifold_edges"] == 0 and np.all(w2 == 1.0)
+    L0 = _L_from_gram(f0.astype(np.int64), a0, g0, n)
+    L2 = _L_from_gram(f2.astype(np.int64), a2, g2, n)
+    assert abs(L2 - L0).max() <= 1e-12 * abs(L0).max()
+    Fo, Lo, Wo, _, _, _ = oi.tufted_delaunay(V, F2)
+    Lr, _, _ = oi.intrinsic_operators(Fo, Lo, n, Wo)
+    assert abs(Lr - L0).max() <= 1e-12 * abs(L0).max()
+
+
+@pytest.mark.parametrize("height,fins", [(5.0, 1), (0.05, 1), (0.08, 2), (3.0, 3)])
+def test_fin_meshes_library_and_oracle_agree(height, fins):
+    """Fins hinged on one edge (3 to 5 faces about it): library and oracle reach the same operators;
+    the Laplacian is symmetric, PSD, annihilates constants, has no positive off-diagonal weight, and
+    the weighted areas add up to the surface's. Low fins force flips across the non-manifold wedges,
+    leaving cover faces of weight 1/2."""
+    V, F = fin_mesh(height=height, fins=fins)
+    n = V.shape[0]
+    faces, lens, w, area, gram, st = _lib.host_intrinsic_cover(V, F)
+    assert st["cover"] and st["nonmanifold_edges"] == 1
+    Fo, Lo, Wo, flips, _, nm = oi.tufted_delaunay(V, F)
+    assert nm == 1 and flips == st["flips"]
+    assert sorted(np.unique(w, return_counts=True)[1].tolist()) == sorted(np.unique(Wo, return_counts=True)[1].tolist())
+    if height < 1.0:
+        assert np.any(w == 0.5)
+    L_lib = _L_from_gram(faces.astype(np.int64), area, gram, n)
+    L_orc, M_orc, a_orc = oi.intrinsic_operators(Fo, Lo, n, Wo)
+    assert abs(L_lib - L_orc).max() <= 1e-10 * abs(L_orc).max()
+    _check_laplacian(L_orc, n)
+    _check_laplacian(L_lib, n)
+    A = _surface_area(V, F)
+    assert abs(area.sum() - A) <= 1e-12 * A and abs(M_orc.diagonal().sum() - A) <= 1e-12 * A
