@@ -50,7 +50,7 @@ class Options(ctypes.Structure):
         ("robust", ctypes.c_int),
         ("split_depth", ctypes.c_int),
         ("band_cap", ctypes.c_int),
-        ("reserved", ctypes.c_int * 1),
+        ("graphs", ctypes.c_int),
     ]
 
 

@@ -178,6 +178,14 @@ struct Workspace {
     std::vector<EventPair> ev_pool;
     std::vector<std::pair<LaunchRec, EventPair>> ev_pending;
     size_t ev_used = 0;
+    // Captured launch sequences replayed for every full block on this stream (fcw_options.graphs):
+    // the heat backward sweep, and divergence + Poisson solve. Their launches and arguments are the
+    // same for every full block of this workspace.
+    struct Captured {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    Captured g_heat_bwd, g_poisson;
 };
 
 struct fcw_ctx {
@@ -233,6 +241,7 @@ struct fcw_ctx {
     // timing
     double stage[6] = {0, 0, 0, 0, 0, 0};
     bool profiling = false;
+    bool no_new_graphs = false;  // set during fcw_wiring_sources_noalloc: no capture may be created
     double ktime[kNumTimed] = {};
     int64_t kcount[kNumTimed] = {};
     std::vector<LaunchRec> launch_log;  // every timed launch while profiling (at most kMaxLaunchLog)
@@ -241,6 +250,10 @@ struct fcw_ctx {
     ~fcw_ctx() {
         for (auto& w : ws) {
             if (w->stream) (void)hipStreamSynchronize(w->stream);
+            for (Workspace::Captured* g : {&w->g_heat_bwd, &w->g_poisson}) {
+                if (g->exec) (void)hipGraphExecDestroy(g->exec);
+                if (g->graph) (void)hipGraphDestroy(g->graph);
+            }
             for (auto& e : w->ev_pool) {
                 (void)hipEventDestroy(e.a);
                 (void)hipEventDestroy(e.b);
@@ -355,6 +368,18 @@ struct fcw_ctx {
 
     void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, double* Dt = nullptr,
                int system = 0) {
+        solve_fwd(w, panel, plan, Y, U, system);
+        solve_bwd(w, panel, Y, Dt, system);
+        if (plan) clear_plan_rows(w, plan, Y, U);
+    }
+
+    void clear_plan_rows(Workspace& w, const BlockPlan* plan, double* Y, double* U) {
+        const int L = sym.nlevels;
+        fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0], ldk,
+                               node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
+    }
+
+    void solve_fwd(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, int system) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
             const fcw::LevelTile* tiles;
@@ -375,6 +400,10 @@ struct fcw_ctx {
                                  (int)sym.urows, plan ? 1 : fwd_split[l], w.part.p);
             ev_end(w, e);
         }
+    }
+
+    void solve_bwd(Workspace& w, const double* panel, double* Y, double* Dt, int system) {
+        const int L = sym.nlevels;
         for (int l = L - 1; l >= 0; --l) {
             EventPair* e = ev_begin(w, kBwdGemm, system, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l]),
                                     level_entries[l]);
@@ -383,9 +412,25 @@ struct fcw_ctx {
                                  bwd_idx_lds, Dt, n, bwd_split[l], w.part.p);
             ev_end(w, e);
         }
-        if (plan)
-            fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
-                                   ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, Y, U);
+    }
+
+    // Run `issue` on w's stream, or, when captured graphs are on for this call, replay its capture
+    // (capturing it the first time).
+    template <class Fn>
+    void run_captured(Workspace& w, Workspace::Captured& g, bool graphable, Fn&& issue) {
+        if (!graphable) {
+            issue();
+            return;
+        }
+        if (!g.exec) {
+            HIPCHK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
+            issue();
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamEndCapture(w.stream, &graph));
+            g.graph = graph;
+            HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        }
+        HIPCHK(hipGraphLaunch(g.exec, w.stream));
     }
 
     // FP32 sweeps of one system: w.X32 -> w.Z32, the solution also stored / added into acc64 (and
@@ -429,6 +474,8 @@ struct fcw_ctx {
     // (n x ldk) and source-major in w.Dt (ldk x n), both unshifted; source j's distances are its
     // column minus its value at its own position.
     void distances(Workspace& w, int k, const BlockPlan* plan) {
+        // full blocks of the default configuration replay the captured sequences
+        const bool graphable = opt.graphs > 0 && !profiling && !no_new_graphs && plan && k == ldk;
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         if (profiling) {
             for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -441,20 +488,35 @@ struct fcw_ctx {
             fcw::launch_clear_rows(w.stream, plan->nodes + plan->node_off[0], plan->node_off[L] - plan->node_off[0],
                                    ldk, node_start.p, node_s.p, node_b.p, node_uoff.p, w.X.p, nullptr);
             fcw::launch_set_delta(w.stream, w.X.p, ldk, w.src_pos.p, k);
-            solve(w, panel_heat.p, plan, w.Yh.p, w.Uh.p, nullptr, 0);
+            solve_fwd(w, panel_heat.p, plan, w.Yh.p, w.Uh.p, 0);
+            run_captured(w, w.g_heat_bwd, graphable, [&] { solve_bwd(w, panel_heat.p, w.Yh.p, nullptr, 0); });
+            clear_plan_rows(w, plan, w.Yh.p, w.Uh.p);
         } else {
             fcw::launch_delta_rhs(w.stream, w.X.p, n, ldk, w.src_pos.p, k);
             solve(w, panel_heat.p, nullptr, w.Yp.p, w.Up.p, nullptr, 0);
         }
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
-        EventPair* ed = ev_begin(w, kDivergence);
-        fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p);
-        ev_end(w, ed);
-        if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
-        if (opt.precision == FCW_FP32_IR)
-            poisson_refined(w);
-        else
-            solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p, 1);
+        auto divergence = [&] {
+            EventPair* ed = ev_begin(w, kDivergence);
+            fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p);
+            ev_end(w, ed);
+        };
+        auto poisson = [&] {
+            if (opt.precision == FCW_FP32_IR)
+                poisson_refined(w);
+            else
+                solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p, 1);
+        };
+        if (graphable) {
+            run_captured(w, w.g_poisson, true, [&] {
+                divergence();
+                poisson();
+            });
+        } else {
+            divergence();
+            if (profiling) HIPCHK(hipEventRecord(ev[2], w.stream));
+            poisson();
+        }
         if (profiling) {
             HIPCHK(hipEventRecord(ev[3], w.stream));
             HIPCHK(hipEventSynchronize(ev[3]));
@@ -854,6 +916,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->sparse_rhs = 1;
     opt->split_depth = 0;
     opt->band_cap = 6144;
+    opt->graphs = 1;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -1298,7 +1361,14 @@ int fcw_wiring_sources_noalloc(fcw_ctx* C, const int* src, int nsrc, fcw_metrics
                                   hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));  // every stream's blocks read these
         plans.bind(dtiles, dnodes);
-        C->run_wiring(dpos, nsrc, plans, ob);
+        C->no_new_graphs = true;  // instantiating a capture allocates; this path issues every launch itself
+        try {
+            C->run_wiring(dpos, nsrc, plans, ob);
+        } catch (...) {
+            C->no_new_graphs = false;
+            throw;
+        }
+        C->no_new_graphs = false;
         C->stage[5] = now_s() - w0;
         C->copy_outputs(ob, (size_t)nsrc, out);
         C->stage[4] = now_s() - w0;

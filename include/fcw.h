@@ -65,7 +65,9 @@ typedef struct {
                               on when streams <= 2 */
     int band_cap;          /* band faces per scale held per source by the metric kernel (default 6144); a band
                               that does not fit is evaluated from every face instead (same result, slower) */
-    int reserved[1];
+    int graphs;            /* 1 (default): each stream captures the launches that are the same for every full
+                              block (the heat backward sweep; divergence + Poisson solve) into HIP graphs once
+                              and replays them; 0: every launch issued one by one */
 } fcw_options;
 
 typedef struct {

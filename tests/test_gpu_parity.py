@@ -286,3 +286,32 @@ def test_analysis_end_to_end_c1(c1):
     assert np.array_equal(an.sample_indptr, ref["sample_indptr"])
     assert np.max(rel_err(an.sample_radii_flat, ref["sample_radii"])) < 1e-5
     assert an.n_flagged_fields == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+def test_graph_replays_match_eager_launches(c2, precision):
+    """Full blocks replay the captured heat backward sweep and divergence + Poisson solve: every
+    replay, over several calls with eager work (partial blocks, another context's launches) and
+    synchronisations between them, gives the eager launches' distances bit for bit and their
+    metrics to 1e-12 (the metric kernel's LDS atomics may add in another order)."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    s, mesh, ctx, H = c2
+    src = np.arange(1, mesh.n, 13)[:600]  # two full 256-source blocks and a partial one
+    with HeatContext(mesh.V, mesh.F, block=256, streams=2, precision=precision, graphs=False) as eager, \
+            HeatContext(mesh.V, mesh.F, block=256, streams=2, precision=precision, graphs=True) as graph:
+        D0 = eager.distances(src)
+        for c in (eager, graph):
+            _set_metrics(c, mesh, _scales())
+        w0 = eager.wiring(src)
+        for rep in range(3):
+            D1 = graph.distances(src)
+            assert np.array_equal(D1, D0), rep
+            w1 = graph.wiring(src)
+            assert np.array_equal(w1["scalars"][:, [0, 7, 8]], w0["scalars"][:, [0, 7, 8]]), rep
+            for key in ("radii", "perimeters", "extra_a"):
+                fin = np.isfinite(w0[key])
+                assert np.array_equal(fin, np.isfinite(w1[key]))
+                assert np.max(rel_err(w1[key][fin], w0[key][fin])) < 1e-12, (rep, key)
+            ctx.distances(src[:37])  # other eager work on the device between the replays
+    print(f"{precision}: 3 rounds of graph replays equal the eager launches", flush=True)
