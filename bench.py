@@ -71,6 +71,11 @@ def prepare(workload):
     return prep
 
 
+def bench_prepare_surface(workload):
+    """Only what the tuner needs of a surface: the submesh and its BFS order."""
+    return prepare(workload)
+
+
 def cpu_sample(n, n_sources, seed=0):
     return np.random.default_rng(seed).choice(n, size=min(n_sources, n), replace=False)
 
@@ -177,11 +182,11 @@ def finish_cpu_leg(leg, n_sources):
 # --------------------------------------------------------------------------------------------
 # GPU measurement
 # --------------------------------------------------------------------------------------------
-def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, check_src, check_d_src):
+def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, check_src, check_d_src, leaf_size=64):
     from fastcorticalwiring_amd.engine import HeatContext
 
     t0 = time.perf_counter()
-    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision, streams=streams)
+    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision, streams=streams, leaf_size=leaf_size)
     ctx.set_metrics(prep["targets"], prep["fracs"], prep["boundary"], prep["vertex_areas"],
                     health_h=prep["health_h"], bin_width=prep["width"])
     setup = time.perf_counter() - t0
@@ -228,7 +233,7 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, chec
     return out
 
 
-def hemisphere_latency(prep, precision, block, streams):
+def hemisphere_latency(prep, precision, block, streams, leaf_size=64):
     """One whole hemisphere from the surface in host memory to every source's outputs in host
     memory: device mesh preparation, analysis and both factorisations, all sources, copy back."""
     from fastcorticalwiring_amd.serving import HemisphereRequest, ServingEngine
@@ -236,7 +241,7 @@ def hemisphere_latency(prep, precision, block, streams):
     s = prep["surface"]
     req = HemisphereRequest(0, 0.0, s.vertices, s.faces, s.cortex_mask)
     eng = ServingEngine(max_active=1, block=block, precision=precision, keep_outputs=False, streams=streams,
-                        chunk_blocks=1 << 20)
+                        chunk_blocks=1 << 20, leaf_size=leaf_size)
     r = eng.run([req])[0]
     if r.error:
         raise RuntimeError(r.error)
@@ -345,7 +350,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="RHS blocks timed (default: every cortical source once)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=3)
-    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--block", type=int, default=256, help="RHS block width when not autotuned")
+    ap.add_argument("--leaf-size", type=int, default=64, help="nested-dissection leaf size when not autotuned")
+    ap.add_argument("--autotune", default="on", choices=("on", "off"),
+                    help="time block widths {128,256,512} x leaf sizes {32,64,96,128} first and use the fastest")
     ap.add_argument("--streams", type=int, default=8, help="RHS blocks in flight (one HIP stream each)")
     ap.add_argument("--precision", default="auto", help="fp64 | fp32_ir | auto (every form)")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
@@ -384,14 +392,25 @@ def main():
     if args.calibrate_bytes:
         _lib.check(_lib.lib().fcw_calibration_copy(int(args.calibrate_bytes), 0))
     forms = ["fp64", "fp32_ir"] if args.precision == "auto" else [args.precision]
+    tuned = None
+    block, leaf = args.block, args.leaf_size
+    if args.autotune == "on":
+        from fastcorticalwiring_amd import autotune
+
+        log("bench: autotuning block width and leaf size")
+        best_cfg, trials = autotune.autotune(prep["V"], prep["F"], blocks=(128, 256, 512), leaf_sizes=(32, 64, 96, 128),
+                                             precision=forms[0], streams=args.streams, order=prep["order"],
+                                             n_sources=min(prep["order"].size, 16384), log=log)
+        block, leaf = best_cfg["block"], best_cfg["leaf_size"]
+        tuned = dict(block=block, leaf_size=leaf, precision=forms[0], trials=trials)
     check_src = cpu_sample(n, args.cpu_sources)  # the CPU leg's sample, checked on the GPU as well
     nw = leg["nworkers"] if leg else 1
     check_d_src = np.concatenate([p[:CPU_KEEP_FIELDS] for p in np.array_split(check_src, nw)])
     results = {}
     for form in forms:
         log(f"bench: timing {form}")
-        results[form] = run_gpu(prep, form, args.block, args.steps, args.warmup, args.profile_blocks,
-                                args.streams, check_src, check_d_src)
+        results[form] = run_gpu(prep, form, block, args.steps, args.warmup, args.profile_blocks,
+                                args.streams, check_src, check_d_src, leaf)
         r = results[form]
         log(f"bench: {form}: {r['sources_per_s']:.1f} sources/s ({r['ms_per_step']:.2f} ms/block), "
             f"setup {r['setup_s']:.2f}s")
@@ -402,24 +421,35 @@ def main():
     latency = {}
     if args.latency == "on":
         for form in forms:
-            latency[form] = hemisphere_latency(prep, form, args.block, args.streams)
+            latency[form] = hemisphere_latency(prep, form, block, args.streams, leaf)
             log(f"bench: {form}: one hemisphere end to end {latency[form]['latency_s']:.2f}s")
     best = max(results.values(), key=lambda r: r["sources_per_s"])
     serving = None
     if args.serving_requests > 0:
         from fastcorticalwiring_amd.serving import ServingEngine, stream_requests
 
+        from fastcorticalwiring_amd import autotune
+        from fastcorticalwiring_amd import meshes as _meshes
+
         log(f"bench: serving stream, {args.serving_requests} requests, {args.serving_precision}")
         reqs = stream_requests(args.serving_requests)
+        # block width and leaf size per size class, tuned before the stream on one surface of each level
+        tuner = autotune.Tuner(precision=args.serving_precision, streams=4, blocks=(256, 512), leaf_sizes=(64, 96),
+                               log=log)
+        for lvl in sorted({r.meta["level"] for r in reqs}):
+            wl = {5: 1, 6: 2, 7: 3}[lvl]
+            sp = prep if wl == args.workload else bench_prepare_surface(wl)
+            tuner.choose(sp["V"], sp["F"], sp["order"])
         def done(r):
             log(f"bench: request {r.rid} ({r.n_sources} sources) done, latency {r.latency_s:.2f}s"
                 + (f", error {r.error}" if r.error else ""))
 
-        eng = ServingEngine(max_active=3, block=args.block, precision=args.serving_precision, keep_outputs=False,
-                            on_done=done)
+        eng = ServingEngine(max_active=3, block=block, precision=args.serving_precision, keep_outputs=False,
+                            on_done=done, tuner=tuner)
         sres = eng.run(reqs, time_scale=args.serving_time_scale)
         serving = ServingEngine.summary(sres)
         serving.update(precision=args.serving_precision, time_scale=args.serving_time_scale,
+                       tuned={str(k): v for k, v in tuner.choices.items()},
                        levels=[r.meta["level"] for r in reqs],
                        stream="folded icospheres L5/L6/L7 (30/40/30%), caps 5-20%, Poisson arrivals mean gap 5 s")
         log(f"bench: serving p50 {serving['p50_s']:.2f}s p95 {serving['p95_s']:.2f}s "
@@ -463,6 +493,7 @@ def main():
             "surface_vertices": int(prep["n_full"]),
             "faces": int(prep["F"].shape[0]),
             "block": int(best["info"]["block"]),
+            "leaf_size": int(leaf),
             "streams": int(args.streams),
             "precision": best["precision"],
             "scales": 12,
@@ -494,6 +525,8 @@ def main():
         "stage_s_per_block": {k: v / best["profile_blocks"] for k, v in best["stage_s"].items()
                               if k not in ("wall", "issue")},
     }
+    if tuned is not None:
+        line["autotune"] = tuned
     if cpu is not None:
         line["cpu_baseline_detail"] = {k: cpu[k] for k in ("potpourri3d", "seconds_per_process")}
     print(json.dumps(line), flush=True)

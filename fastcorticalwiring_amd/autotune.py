@@ -1,32 +1,59 @@
 """Autotuner for the solve configuration of one mesh.
 
-Times a fixed sample of sources (the first ``n_sources`` of the BFS order,
-identical for every candidate) through the full per-source pipeline for each
+Times the same sample of sources (a contiguous run of the BFS order, identical
+for every candidate) through the whole per-source pipeline for each
 (RHS block width, nested-dissection leaf size) pair and returns the fastest.
-The leaf size sets the supernode shapes (leaf panel (s+b) x s with s <= leaf);
-the block width sets the panel-product N dimension and the number of sources
-per launch.
+The leaf size sets the supernode shapes (leaf panels (s+b) x s with s <= leaf,
+and how much update-row traffic the levels above the leaves carry); the block
+width sets the panel products' N dimension and the number of sources per
+launch. With several streams the sample is long enough to keep every stream
+busy (16 blocks of the widest candidate at least), so the rate measured is the
+throughput rate, not one block's latency.
+
+``Tuner`` keeps the choice per mesh size (vertex count rounded to a power of
+two), which is how the serving loop uses it: the first hemisphere of a size
+class pays for the trials, later ones reuse the result.
 """
 
 from __future__ import annotations
 
 import itertools
+import math
+import threading
 import time
 
 import numpy as np
 
 from . import mesh_prep as mp
 from . import meshes
-from .engine import HeatContext
+
+DEFAULT_BLOCKS = (128, 256, 512)
+DEFAULT_LEAF_SIZES = (32, 64, 128)
 
 
-def autotune(vertices, faces, blocks=(128, 256, 512), leaf_sizes=(32, 64, 128), n_sources=2048, scales=None,
-             precision="fp64", repeats=2, log=None):
+def _default_context(V, F, block, leaf_size, precision, streams):
+    from .engine import HeatContext
+
+    return HeatContext(V, F, block=block, leaf_size=leaf_size, precision=precision, streams=streams)
+
+
+def autotune(vertices, faces, blocks=DEFAULT_BLOCKS, leaf_sizes=DEFAULT_LEAF_SIZES, n_sources=None, scales=None,
+             precision="fp64", streams=8, repeats=2, log=None, make_context=None, order=None):
+    """Time every (block, leaf size) candidate; returns (best, trials). Each trial records the
+    sources/s over the sample, the context's setup seconds and its factor size.
+
+    make_context(V, F, block, leaf_size, precision, streams) builds the context (the device one by
+    default); the timing only needs its ``block``, ``info``, ``set_metrics``, ``wiring`` and ``close``.
+    """
     V = np.ascontiguousarray(vertices, dtype=np.float64)
     F = np.ascontiguousarray(faces, dtype=np.int32)
+    make_context = make_context or _default_context
     fa, fl, va = mp.face_geometry(V, F)
-    order = mp.bfs_order_all(V.shape[0], F)
-    src = np.resize(order, max(n_sources, max(blocks)))
+    if order is None:
+        order = mp.bfs_order_all(V.shape[0], F)
+    if n_sources is None:
+        n_sources = 16 * max(blocks) * max(1, streams) // 8
+    src = np.resize(np.asarray(order), max(int(n_sources), 2 * max(blocks)))
     T = np.array(sorted(scales or meshes.TWELVE_SCALES)) * float(va.sum())
     fr = np.array([0.25, 0.5, 0.75])
     bnd = mp.boundary_indices(F)
@@ -34,20 +61,55 @@ def autotune(vertices, faces, blocks=(128, 256, 512), leaf_sizes=(32, 64, 128), 
     trials = []
     for leaf, blk in itertools.product(leaf_sizes, blocks):
         t0 = time.perf_counter()
-        ctx = HeatContext(V, F, block=blk, leaf_size=leaf, precision=precision)
+        ctx = make_context(V, F, blk, leaf, precision, streams)
         setup = time.perf_counter() - t0
-        ctx.set_metrics(T, fr, bnd, va, health_h=hh, bin_width=w)
-        ctx.wiring(src[:blk])  # warm-up
-        best = float("inf")
-        for _ in range(repeats):
-            t0 = time.perf_counter()
-            ctx.wiring(src)
-            best = min(best, time.perf_counter() - t0)
-        rate = src.size / best
-        trials.append(dict(block=blk, leaf_size=leaf, sources_per_s=rate, setup_s=setup,
-                           nnodes=ctx.info["nnodes"], factor_nnz=ctx.info["factor_nnz"]))
+        try:
+            ctx.set_metrics(T, fr, bnd, va, health_h=hh, bin_width=w)
+            ctx.wiring(src[: 2 * ctx.block])  # warm-up
+            best = float("inf")
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                ctx.wiring(src)
+                best = min(best, time.perf_counter() - t0)
+            rate = src.size / best
+            trials.append(dict(block=blk, leaf_size=leaf, sources_per_s=rate, setup_s=setup,
+                               nnodes=int(ctx.info["nnodes"]), factor_nnz=int(ctx.info["factor_nnz"])))
+        finally:
+            ctx.close()
         if log:
             log(f"autotune: block {blk:4d} leaf {leaf:4d}: {rate:10.1f} sources/s (setup {setup:.2f}s)")
-        ctx.close()
     best = max(trials, key=lambda t: t["sources_per_s"])
     return best, trials
+
+
+def size_class(n):
+    """Meshes whose vertex counts round to the same power of two share a tuned configuration."""
+    return int(round(math.log2(max(int(n), 1))))
+
+
+class Tuner:
+    """Tuned (block, leaf size) per mesh size class, computed on first use and cached."""
+
+    def __init__(self, precision="fp64", streams=8, blocks=DEFAULT_BLOCKS, leaf_sizes=DEFAULT_LEAF_SIZES, log=None,
+                 make_context=None, n_sources=None):
+        self.precision = precision
+        self.streams = streams
+        self.blocks = tuple(blocks)
+        self.leaf_sizes = tuple(leaf_sizes)
+        self.log = log
+        self.make_context = make_context
+        self.n_sources = n_sources
+        self._lock = threading.Lock()
+        self.choices = {}
+        self.trials = {}
+
+    def choose(self, vertices, faces, order=None):
+        key = size_class(np.asarray(vertices).shape[0])
+        with self._lock:  # one hemisphere of a class tunes, the others wait for its result
+            if key not in self.choices:
+                best, trials = autotune(vertices, faces, self.blocks, self.leaf_sizes, self.n_sources,
+                                        precision=self.precision, streams=self.streams, log=self.log,
+                                        make_context=self.make_context, order=order)
+                self.choices[key] = dict(block=best["block"], leaf_size=best["leaf_size"])
+                self.trials[key] = trials
+            return dict(self.choices[key])
