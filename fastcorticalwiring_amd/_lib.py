@@ -29,6 +29,7 @@ except ValueError:
 
 FCW_FP64 = 0
 FCW_FP32_IR = 1
+FCW_BF16X3_IR = 2
 
 (S_FLAG, S_NEG_FRAC, S_VIOL_FRAC, S_DIST_BOUNDARY, S_MSD_U, S_MSD_W, S_N_EVAL, S_N_SAMPLES,
  S_OVERFLOW, NSCALARS) = range(10)

@@ -24,6 +24,10 @@
 #include "fcw_device.h"
 #include "fcw_internal.h"
 
+namespace fcw {
+thread_local GraphRecorder* t_recorder = nullptr;
+}
+
 namespace {
 
 thread_local std::string g_err;
@@ -69,6 +73,12 @@ struct DBuf {
     void upload(const T* h, size_t count, int64_t& tally) {
         alloc(count, tally);
         if (count) HIPCHK(hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    // Per-call buffers: reallocated only to grow (a free can wait for every stream of the device,
+    // other contexts' included), filled on `st`, which the caller synchronises or orders after.
+    void refill(const T* h, size_t count, int64_t& tally, hipStream_t st) {
+        if (count > n || !p) alloc(std::max<size_t>(count, 1), tally);
+        if (count) HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
     }
 };
 
@@ -172,7 +182,7 @@ struct Workspace {
     DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
     DBuf<double> part;  // partial products of the split-depth sweep launches
     DBuf<double> cpart;              // chunk_stats records (k x nch x kChunkStats)
-    DBuf<float> X32, Y32, U32, Z32;  // refined solve (FCW_FP32_IR): FP32 right-hand side and sweeps
+    DBuf<float> X32, Y32, U32, Z32;  // refined solves: FP32 right-hand side and sweeps
     DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
     DBuf<int> src_pos;
     std::vector<EventPair> ev_pool;
@@ -220,10 +230,12 @@ struct fcw_ctx {
     bool fwd_idx_lds = true, bwd_idx_lds = true;  // every node's sweep indices fit in LDS
     DBuf<double> panel_heat, panel_pois;
     DBuf<float> panel_pois32;  // FCW_FP32_IR: the Poisson panels rounded to FP32
+    DBuf<uint32_t> panel_pois_bf;  // FCW_BF16X3_IR: the Poisson panels split into bf16 hi | lo << 16
     DBuf<double> Dout;
     // per-call plans
     DBuf<fcw::LevelTile> plan_tiles;
     DBuf<int> plan_nodes;
+    DBuf<int> call_pos;  // the current call's source positions
     // 256-face chunks: vertices (owned first, cnown of them), area
     int nch = 0;
     DBuf<int> cvptr, cverts, cnown, isb;
@@ -341,7 +353,7 @@ struct fcw_ctx {
         w.Up.alloc(urows * ld, dev_bytes);
         w.Dt.alloc((size_t)n * ld, dev_bytes);
         w.src_pos.alloc(ld, dev_bytes);
-        if (opt.precision == FCW_FP32_IR) {
+        if (opt.precision != FCW_FP64) {
             w.X32.alloc((size_t)n * ld, dev_bytes);
             w.Y32.alloc((size_t)n * ld, dev_bytes);
             w.Z32.alloc((size_t)n * ld, dev_bytes);
@@ -414,8 +426,9 @@ struct fcw_ctx {
         }
     }
 
-    // Run `issue` on w's stream, or, when captured graphs are on for this call, replay its capture
-    // (capturing it the first time).
+    // Run `issue` on w's stream, or, when graphs are on for this call, launch its graph (built the
+    // first time: the launches `issue` makes are recorded as a chain of kernel nodes instead of
+    // being started, so no stream is captured).
     template <class Fn>
     void run_captured(Workspace& w, Workspace::Captured& g, bool graphable, Fn&& issue) {
         if (!graphable) {
@@ -423,10 +436,19 @@ struct fcw_ctx {
             return;
         }
         if (!g.exec) {
-            HIPCHK(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal));
-            issue();
             hipGraph_t graph = nullptr;
-            HIPCHK(hipStreamEndCapture(w.stream, &graph));
+            HIPCHK(hipGraphCreate(&graph, 0));
+            fcw::GraphRecorder rec;
+            rec.graph = graph;
+            fcw::t_recorder = &rec;
+            try {
+                issue();
+            } catch (...) {
+                fcw::t_recorder = nullptr;
+                (void)hipGraphDestroy(graph);
+                throw;
+            }
+            fcw::t_recorder = nullptr;
             g.graph = graph;
             HIPCHK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
         }
@@ -435,7 +457,8 @@ struct fcw_ctx {
 
     // FP32 sweeps of one system: w.X32 -> w.Z32, the solution also stored / added into acc64 (and
     // written source-major into Dt when set).
-    void solve32(Workspace& w, const float* panel, double* acc64, int acc_mode, double* Dt = nullptr) {
+    template <typename PT>
+    void solve32(Workspace& w, const PT* panel, double* acc64, int acc_mode, double* Dt = nullptr) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
             EventPair* e = ev_begin(w, kFwd32, 1, l, sweep_wgs(fwd_tile_off[l + 1] - fwd_tile_off[l]), level_entries[l]);
@@ -456,17 +479,25 @@ struct fcw_ctx {
     // Poisson system by refinement: an FP32 solve of the right-hand side, then ir_steps times the
     // FP64 residual r = b - A x rounded to FP32 and an FP32 solve of A dx = r added to x (FP64).
     void poisson_refined(Workspace& w) {
+        if (opt.precision == FCW_BF16X3_IR)
+            poisson_refined_with(w, panel_pois_bf.p);
+        else
+            poisson_refined_with(w, panel_pois32.p);
+    }
+
+    template <typename PT>
+    void poisson_refined_with(Workspace& w, const PT* panel) {
         EventPair* e = ev_begin(w, kResidual, 1);
         fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p, nullptr,
                                w.X32.p);
         ev_end(w, e);
-        solve32(w, panel_pois32.p, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
+        solve32(w, panel, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
             e = ev_begin(w, kResidual, 1);
             fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p,
                                    w.Z.p, w.X32.p);
             ev_end(w, e);
-            solve32(w, panel_pois32.p, w.Z.p, 2, it + 1 == opt.ir_steps ? w.Dt.p : nullptr);
+            solve32(w, panel, w.Z.p, 2, it + 1 == opt.ir_steps ? w.Dt.p : nullptr);
         }
     }
 
@@ -502,7 +533,7 @@ struct fcw_ctx {
             ev_end(w, ed);
         };
         auto poisson = [&] {
-            if (opt.precision == FCW_FP32_IR)
+            if (opt.precision != FCW_FP64)
                 poisson_refined(w);
             else
                 solve(w, panel_pois.p, nullptr, w.Yp.p, w.Up.p, w.Dt.p, 1);
@@ -570,11 +601,18 @@ struct fcw_ctx {
         return cp;
     }
 
-    // Plans uploaded to the context's own plan arrays.
-    CallPlans make_plans(const std::vector<int>& pos) {
+    // Plans (and, with `dpos`, the source positions) uploaded to the context's own per-call arrays;
+    // the copies are complete when this returns.
+    CallPlans make_plans(const std::vector<int>& pos, const int** dpos = nullptr) {
         CallPlans cp = plan_blocks(pos);
-        plan_tiles.upload(cp.tiles.data(), std::max<size_t>(1, cp.tiles.size()), dev_bytes);
-        plan_nodes.upload(cp.nodes.data(), std::max<size_t>(1, cp.nodes.size()), dev_bytes);
+        hipStream_t st = ws[0]->stream;
+        plan_tiles.refill(cp.tiles.data(), cp.tiles.size(), dev_bytes, st);
+        plan_nodes.refill(cp.nodes.data(), cp.nodes.size(), dev_bytes, st);
+        if (dpos) {
+            call_pos.refill(pos.data(), pos.size(), dev_bytes, st);
+            *dpos = call_pos.p;
+        }
+        HIPCHK(hipStreamSynchronize(st));
         cp.bind(plan_tiles.p, plan_nodes.p);
         return cp;
     }
@@ -678,8 +716,10 @@ struct fcw_ctx {
     void copy_outputs(const OutBase& b, size_t k, fcw_metrics_out* out) {
         sync_all();
         const size_t NE = S > 0 ? (size_t)(S - 1) * q : 0;
+        // copies ordered on the context's own stream (not the legacy stream)
+        hipStream_t st = ws[0]->stream;
         auto cp = [&](void* dst, const void* src, size_t bytes) {
-            if (dst && bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+            if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
         };
         cp(out->scalars, b.scal, k * fcw::kSlotCount * sizeof(double));
         cp(out->radii, b.radii, k * S * sizeof(double));
@@ -688,6 +728,7 @@ struct fcw_ctx {
         cp(out->extra_a, b.ea, k * NE * sizeof(double));
         cp(out->sample_r, b.sr, k * (S + NE) * sizeof(float));
         cp(out->sample_a, b.sa, k * (S + NE) * sizeof(float));
+        HIPCHK(hipStreamSynchronize(st));
     }
 
     // The blocks of one wiring call: source positions `dpos` (device, nsrc), plans bound to device
@@ -941,7 +982,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         fcw_options opt;
         fcw_default_options(&opt);
         if (opt_in) opt = *opt_in;
-        if (opt.precision != FCW_FP64 && opt.precision != FCW_FP32_IR) throw Fail("unknown precision");
+        if (opt.precision != FCW_FP64 && opt.precision != FCW_FP32_IR && opt.precision != FCW_BF16X3_IR)
+            throw Fail("unknown precision");
         if (opt.ir_steps < 0 || opt.ir_steps > 8) throw Fail("ir_steps must be in 0..8");
         if (opt.block <= 0) opt.block = 256;
         if (opt.streams <= 0) opt.streams = 1;
@@ -1111,6 +1153,10 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         if (opt.precision == FCW_FP32_IR) {
             C->panel_pois32.alloc(pp.size(), tb);
             fcw::launch_to_float(st, C->panel_pois.p, C->panel_pois32.p, (int64_t)pp.size());
+        }
+        if (opt.precision == FCW_BF16X3_IR) {
+            C->panel_pois_bf.alloc(pp.size(), tb);
+            fcw::launch_to_bf16x2(st, C->panel_pois.p, C->panel_pois_bf.p, (int64_t)pp.size());
         }
         C->node_start.upload(S.node_start.data(), S.node_start.size(), tb);
         C->node_s.upload(S.node_s.data(), S.node_s.size(), tb);
@@ -1309,12 +1355,10 @@ int fcw_wiring_sources(fcw_ctx* C, const int* src, int nsrc, fcw_metrics_out* ou
         if (nsrc <= 0) return;
         const double w0 = now_s();
         std::vector<int> pos = source_positions(C, src, nsrc);
-        DBuf<int> all_pos;
-        int64_t scratch_bytes = 0;
-        all_pos.upload(pos.data(), pos.size(), scratch_bytes);
-        CallPlans plans = C->make_plans(pos);
+        const int* dpos = nullptr;
+        CallPlans plans = C->make_plans(pos, &dpos);
         C->ensure_outputs((size_t)nsrc);
-        C->run_wiring(all_pos.p, nsrc, plans, C->own_outputs());
+        C->run_wiring(dpos, nsrc, plans, C->own_outputs());
         C->stage[5] = now_s() - w0;
         C->copy_outputs(C->own_outputs(), (size_t)nsrc, out);
         for (auto& w : C->ws) C->ev_collect(*w);

@@ -3,11 +3,53 @@
 
 #include <hip/hip_runtime.h>
 
-#include <string>
-
 #include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <utility>
 
 namespace fcw {
+
+// Every kernel of the library is started through launch(): issued on the stream, or, while a
+// GraphRecorder is active on the calling thread, appended to the recorder's graph as the next node
+// of one chain (each node depends on the one before). Graphs are built this way rather than by
+// stream capture, so no stream ever enters capture mode and no other thread's work (in this
+// library or not) is affected while a graph is being built.
+struct GraphRecorder {
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+};
+extern thread_local GraphRecorder* t_recorder;
+
+template <class Tuple, std::size_t... I>
+inline void kernel_arg_ptrs(Tuple& t, void** ptrs, std::index_sequence<I...>) {
+    ((ptrs[I] = static_cast<void*>(&std::get<I>(t))), ...);
+}
+
+template <class... KArgs, class... Args>
+inline void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem, hipStream_t st, Args&&... args) {
+    static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+    if (!t_recorder) {
+        kernel<<<grid, block, shmem, st>>>(std::forward<Args>(args)...);
+        return;
+    }
+    std::tuple<KArgs...> vals(std::forward<Args>(args)...);  // the kernel's own parameter types
+    void* ptrs[sizeof...(KArgs) > 0 ? sizeof...(KArgs) : 1];
+    kernel_arg_ptrs(vals, ptrs, std::index_sequence_for<KArgs...>{});
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(kernel);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = (unsigned)shmem;
+    p.kernelParams = ptrs;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&node, t_recorder->graph, t_recorder->last ? &t_recorder->last : nullptr,
+                                               t_recorder->last ? 1 : 0, &p);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipGraphAddKernelNode failed: ") + hipGetErrorString(e));
+    t_recorder->last = node;
+}
 
 struct LevelTile {
     int node;
@@ -180,6 +222,17 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
                      double* acc64, int acc_mode, double* Dt, int ldd);
+// Split-BF16 forms (refined solve, FCW_BF16X3_IR): panels packed as bf16 hi | lo << 16 (from the FP64
+// panels by launch_to_bf16x2), FP32 right-hand sides split while staged, three bf16 products per block.
+void launch_to_bf16x2(hipStream_t st, const double* src, uint32_t* dst, int64_t count);
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+                     const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
+                     const int* contrib_tail, const uint32_t* panel, const float* X, float* Y, float* U, bool idx_lds,
+                     bool leaves, int zrow);
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+                     const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff,
+                     const uint32_t* panel, const float* Y, float* Z, bool idx_lds, double* acc64, int acc_mode,
+                     double* Dt, int ldd);
 // r = b - (L + shift M) x in FP64 (x null: r = b), rounded to FP32 (the refined solve's RHS).
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r);

@@ -372,46 +372,46 @@ inline dim3 grid1(int64_t n, int bs) { return dim3((unsigned)((n + bs - 1) / bs)
 
 void launch_face_geometry(hipStream_t st, int m, const double* V, const int* faces, double* area, double* gram,
                           double* larea, double* g11, double* g12, double* g22) {
-    face_geometry_kernel<<<grid1(m, 256), 256, 0, st>>>(m, V, faces, area, gram, larea, g11, g12, g22);
+    fcw::launch(face_geometry_kernel, dim3(grid1(m, 256)), dim3(256), 0, st, m, V, faces, area, gram, larea, g11, g12, g22);
 }
 
 void launch_assemble(hipStream_t st, int n, int nnz, const int* contrib_ptr, const int* contrib, const double* area,
                      const double* gram, const int* vf_ptr, const int* vf, double* lval, double* mass) {
-    assemble_l_kernel<<<grid1(nnz, 256), 256, 0, st>>>(nnz, contrib_ptr, contrib, area, gram, lval);
-    lumped_mass_kernel<<<grid1(n, 256), 256, 0, st>>>(n, vf_ptr, vf, area, mass);
+    fcw::launch(assemble_l_kernel, dim3(grid1(nnz, 256)), dim3(256), 0, st, nnz, contrib_ptr, contrib, area, gram, lval);
+    fcw::launch(lumped_mass_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, n, vf_ptr, vf, area, mass);
 }
 
 void launch_fill(hipStream_t st, double* p, int64_t count, double value) {
     if (count <= 0) return;
     int64_t blocks = (count + 255) / 256;
     if (blocks > 8192) blocks = 8192;
-    fill_kernel<<<dim3((unsigned)blocks), 256, 0, st>>>(p, count, value);
+    fcw::launch(fill_kernel, dim3(dim3((unsigned)blocks)), dim3(256), 0, st, p, count, value);
 }
 
 void launch_calibration_copy(hipStream_t st, const void* src, void* dst, int64_t bytes) {
     const int64_t count = bytes / (int64_t)sizeof(double4);
-    calibration_copy_kernel<<<dim3(4096), 256, 0, st>>>((const double4*)src, (double4*)dst, count);
+    fcw::launch(calibration_copy_kernel, dim3(dim3(4096)), dim3(256), 0, st, (const double4*)src, (double4*)dst, count);
 }
 
 void launch_set_delta(hipStream_t st, double* X, int ldk, const int* src_pos, int k) {
-    delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
+    fcw::launch(delta_rhs_kernel, dim3(grid1(k, 256)), dim3(256), 0, st, X, ldk, src_pos, k);
 }
 
 void launch_delta_rhs(hipStream_t st, double* X, int n, int ldk, const int* src_pos, int k) {
     launch_fill(st, X, (int64_t)n * ldk, 0.0);
-    delta_rhs_kernel<<<grid1(k, 256), 256, 0, st>>>(X, ldk, src_pos, k);
+    fcw::launch(delta_rhs_kernel, dim3(grid1(k, 256)), dim3(256), 0, st, X, ldk, src_pos, k);
 }
 
 void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
                        const int* node_s, const int* node_b, const int64_t* node_uoff, double* Y, double* U) {
     if (count <= 0) return;
-    clear_rows_kernel<<<dim3(count, ldk / 64), 256, 0, st>>>(nodes, ldk, node_start, node_s, node_b, node_uoff, Y, U);
+    fcw::launch(clear_rows_kernel, dim3(dim3(count, ldk / 64)), dim3(256), 0, st, nodes, ldk, node_start, node_s, node_b, node_uoff, Y, U);
 }
 
 void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const double* area, const double* gram,
                                double* geo) {
     if (ninc <= 0) return;
-    incidence_geometry_kernel<<<grid1(ninc, 256), 256, 0, st>>>(ninc, vf, area, gram, geo);
+    fcw::launch(incidence_geometry_kernel, dim3(grid1(ninc, 256)), dim3(256), 0, st, ninc, vf, area, gram, geo);
 }
 
 void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out) {
@@ -419,22 +419,22 @@ void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u
     const int ncs = ldk / kDivSlice;
     const int nitems = d.ngroups * ncs;
     const int run = (nitems + kXcds - 1) / kXcds;  // items per XCD
-    divergence_kernel<<<dim3(run * kXcds), 256, 0, st>>>(nitems, ncs, run, d, u, out, ldk);
+    fcw::launch(divergence_kernel, dim3(dim3(run * kXcds)), dim3(256), 0, st, nitems, ncs, run, d, u, out, ldk);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r) {
-    residual32_kernel<<<dim3((n + 3) / 4, ldk / 64), 256, 0, st>>>(n, ldk, rowptr, colind, lval, mass, shift, b, x, r);
+    fcw::launch(residual32_kernel, dim3(dim3((n + 3) / 4, ldk / 64)), dim3(256), 0, st, n, ldk, rowptr, colind, lval, mass, shift, b, x, r);
 }
 
 void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count) {
     if (count <= 0) return;
-    to_float_kernel<<<grid1(count, 256), 256, 0, st>>>(src, dst, count);
+    fcw::launch(to_float_kernel, dim3(grid1(count, 256)), dim3(256), 0, st, src, dst, count);
 }
 
 void launch_gather_rows(hipStream_t st, int n, int ldk, int k, const double* Dt, const double* phi, const int* src_pos,
                         const int* iperm, double* out) {
-    gather_rows_kernel<<<dim3((n + 255) / 256, k), 256, 0, st>>>(n, ldk, k, Dt, phi, src_pos, iperm, out);
+    fcw::launch(gather_rows_kernel, dim3(dim3((n + 255) / 256, k)), dim3(256), 0, st, n, ldk, k, Dt, phi, src_pos, iperm, out);
 }
 
 

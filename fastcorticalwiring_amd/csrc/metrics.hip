@@ -1167,21 +1167,21 @@ __global__ __launch_bounds__(256) void chunk_stats_kernel(int ldk, int k, int nc
 void launch_chunk_area(hipStream_t st, int m, const double* larea, double* chunk_area) {
     const int nch = (m + kWave * UNR - 1) / (kWave * UNR);
     if (nch <= 0) return;
-    chunk_area_kernel<<<dim3((nch + 255) / 256), 256, 0, st>>>(m, larea, chunk_area);
+    fcw::launch(chunk_area_kernel, dim3(dim3((nch + 255) / 256)), dim3(256), 0, st, m, larea, chunk_area);
 }
 
 void launch_chunk_stats(hipStream_t st, int k, int ldk, int nch, const double* Z, const int* src_pos, const double* V,
                         const double* vw, const int* isb, const int* cvptr, const int* cverts, const int* cnown,
                         double eps, double h, float* ch_lo, float* ch_hi, double* part) {
     if (k <= 0 || nch <= 0) return;
-    chunk_stats_kernel<<<dim3(nch, (k + 255) / 256), 256, 0, st>>>(ldk, k, nch, Z, src_pos, V, vw, isb, cvptr, cverts,
+    fcw::launch(chunk_stats_kernel, dim3(dim3(nch, (k + 255) / 256)), dim3(256), 0, st, ldk, k, nch, Z, src_pos, V, vw, isb, cvptr, cverts,
                                                                    cnown, eps, h, ch_lo, ch_hi, part);
 }
 
 void launch_metrics(hipStream_t st, int k, const MeshDev& mesh, const MetricsDev& md, const double* Dt,
                     const int* src_pos, MetricsOut out, bool health_screen) {
     if (k <= 0) return;
-    metrics_kernel<<<dim3(k), NT, 0, st>>>(mesh, md, Dt, src_pos, out, health_screen ? 1 : 0);
+    fcw::launch(metrics_kernel, dim3(dim3(k)), dim3(NT), 0, st, mesh, md, Dt, src_pos, out, health_screen ? 1 : 0);
 }
 
 }  // namespace fcw

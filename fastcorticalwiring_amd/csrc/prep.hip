@@ -319,9 +319,9 @@ void check(hipError_t e, const char* what) {
 // out = exclusive scan of in[0:n]; returns the total. sums holds one int per 1024 elements.
 int scan(hipStream_t st, const int* in, int n, int* out, int* total_dev, int* sums) {
     const int nb = std::max(1, (n + kScanBlock - 1) / kScanBlock);
-    scan_block_sums<<<nb, kScanBlock, 0, st>>>(in, n, sums);
-    scan_sums_serial<<<1, 1, 0, st>>>(sums, nb, total_dev);
-    scan_apply<<<nb, kScanBlock, 0, st>>>(in, n, sums, out);
+    fcw::launch(scan_block_sums, dim3(nb), dim3(kScanBlock), 0, st, in, n, sums);
+    fcw::launch(scan_sums_serial, dim3(1), dim3(1), 0, st, sums, nb, total_dev);
+    fcw::launch(scan_apply, dim3(nb), dim3(kScanBlock), 0, st, in, n, sums, out);
     int total = 0;
     check(hipMemcpyAsync(&total, total_dev, sizeof(int), hipMemcpyDeviceToHost, st), "scan total");
     check(hipStreamSynchronize(st), "scan");
@@ -349,60 +349,60 @@ int mesh_prep_device(int n, const double* Vh, int m, const int* Fh, const unsign
         check(hipMemcpyAsync(mask.p, maskh, n, hipMemcpyHostToDevice, st), "upload mask");
         Tmp<int> total(1), keep(m), fpos(m), used(n), vpos(n);
         Tmp<int> sums((std::max(n, m) + 1) / kScanBlock + 2);
-        keep_faces_kernel<<<blocks_for(m, bs), bs, 0, st>>>(m, F.p, V.p, mask.p, keep.p);
+        fcw::launch(keep_faces_kernel, dim3(blocks_for(m, bs)), dim3(bs), 0, st, m, F.p, V.p, mask.p, keep.p);
         const int ms = scan(st, keep.p, m, fpos.p, total.p, sums.p);
-        fill_int_kernel<<<blocks_for(n, bs), bs, 0, st>>>(used.p, n, 0);
-        mark_used_kernel<<<blocks_for(m, bs), bs, 0, st>>>(m, F.p, keep.p, used.p);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(n, bs)), dim3(bs), 0, st, used.p, n, 0);
+        fcw::launch(mark_used_kernel, dim3(blocks_for(m, bs)), dim3(bs), 0, st, m, F.p, keep.p, used.p);
         const int ns = scan(st, used.p, n, vpos.p, total.p, sums.p);
         out.n_sub = ns;
         out.m_sub = ms;
         Tmp<int> s2o(ns), o2s(n), Fs(3 * (size_t)ms);
         Tmp<double> Vs(3 * (size_t)ns);
-        vertex_maps_kernel<<<blocks_for(n, bs), bs, 0, st>>>(n, used.p, vpos.p, V.p, s2o.p, o2s.p, Vs.p);
-        sub_faces_kernel<<<blocks_for(m, bs), bs, 0, st>>>(m, F.p, keep.p, fpos.p, o2s.p, Fs.p);
+        fcw::launch(vertex_maps_kernel, dim3(blocks_for(n, bs)), dim3(bs), 0, st, n, used.p, vpos.p, V.p, s2o.p, o2s.p, Vs.p);
+        fcw::launch(sub_faces_kernel, dim3(blocks_for(m, bs)), dim3(bs), 0, st, m, F.p, keep.p, fpos.p, o2s.p, Fs.p);
         // submesh geometry and per-vertex lists
         Tmp<double> areas(ms), L(ms), va(ns);
-        face_geometry_prep_kernel<<<blocks_for(ms, bs), bs, 0, st>>>(ms, Fs.p, Vs.p, areas.p, L.p);
+        fcw::launch(face_geometry_prep_kernel, dim3(blocks_for(ms, bs)), dim3(bs), 0, st, ms, Fs.p, Vs.p, areas.p, L.p);
         Tmp<int> ic(ns + 1), hc(ns + 1), ioff(ns + 1), hoff(ns + 1), icur(ns), hcur(ns);
-        fill_int_kernel<<<blocks_for(ns + 1, bs), bs, 0, st>>>(ic.p, ns + 1, 0);
-        fill_int_kernel<<<blocks_for(ns + 1, bs), bs, 0, st>>>(hc.p, ns + 1, 0);
-        fill_int_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(icur.p, ns, 0);
-        fill_int_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(hcur.p, ns, 0);
-        count_incidences_kernel<<<blocks_for(ms, bs), bs, 0, st>>>(ms, Fs.p, ic.p, hc.p);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(ns + 1, bs)), dim3(bs), 0, st, ic.p, ns + 1, 0);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(ns + 1, bs)), dim3(bs), 0, st, hc.p, ns + 1, 0);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, icur.p, ns, 0);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, hcur.p, ns, 0);
+        fcw::launch(count_incidences_kernel, dim3(blocks_for(ms, bs)), dim3(bs), 0, st, ms, Fs.p, ic.p, hc.p);
         const int ni = scan(st, ic.p, ns + 1, ioff.p, total.p, sums.p);
         const int nh = scan(st, hc.p, ns + 1, hoff.p, total.p, sums.p);
         Tmp<int> ikey(ni), hnb(nh), ucnt(ns + 1), uoff(ns + 1), bflag(ns), bpos(ns);
-        fill_lists_kernel<<<blocks_for(ms, bs), bs, 0, st>>>(ms, Fs.p, ioff.p, icur.p, ikey.p, hoff.p, hcur.p, hnb.p);
-        fill_int_kernel<<<1, 1, 0, st>>>(ucnt.p + ns, 1, 0);
-        vertex_lists_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(ns, ms, ioff.p, ikey.p, areas.p, va.p, hoff.p, hnb.p,
+        fcw::launch(fill_lists_kernel, dim3(blocks_for(ms, bs)), dim3(bs), 0, st, ms, Fs.p, ioff.p, icur.p, ikey.p, hoff.p, hcur.p, hnb.p);
+        fcw::launch(fill_int_kernel, dim3(1), dim3(1), 0, st, ucnt.p + ns, 1, 0);
+        fcw::launch(vertex_lists_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, ns, ms, ioff.p, ikey.p, areas.p, va.p, hoff.p, hnb.p,
                                                                ucnt.p, bflag.p);
         const int nu = scan(st, ucnt.p, ns + 1, uoff.p, total.p, sums.p);
         Tmp<int> cols(nu), bnd(ns);
-        unique_neighbours_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(ns, hoff.p, hnb.p, uoff.p, cols.p);
+        fcw::launch(unique_neighbours_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, ns, hoff.p, hnb.p, uoff.p, cols.p);
         const int nb = scan(st, bflag.p, ns, bpos.p, total.p, sums.p);
-        compact_flags_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(ns, bflag.p, bpos.p, bnd.p);
+        fcw::launch(compact_flags_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, ns, bflag.p, bpos.p, bnd.p);
         out.n_boundary = nb;
         out.n_adj = nu;
         // breadth-first order over every component
         Tmp<int> order(ns), visited(ns), cnt(std::max(1, ns)), off(std::max(1, ns)), best(1);
         Tmp<unsigned long long> claim(ns);
-        fill_int_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(visited.p, ns, 0);
-        fill_u64_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(claim.p, ns, ~0ull);
+        fcw::launch(fill_int_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, visited.p, ns, 0);
+        fcw::launch(fill_u64_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, claim.p, ns, ~0ull);
         int placed = 0;
         int s = (start >= 0 && start < ns) ? start : 0;
         while (placed < ns) {
-            start_component_kernel<<<1, 1, 0, st>>>(order.p, placed, s, visited.p, claim.p);
+            fcw::launch(start_component_kernel, dim3(1), dim3(1), 0, st, order.p, placed, s, visited.p, claim.p);
             int f0 = placed, fn = 1;
             placed += 1;
             while (fn > 0) {
-                bfs_claim_kernel<<<blocks_for(fn, bs), bs, 0, st>>>(order.p, f0, fn, uoff.p, cols.p, visited.p, claim.p);
-                bfs_count_kernel<<<blocks_for(fn, bs), bs, 0, st>>>(order.p, f0, fn, uoff.p, cols.p, visited.p, claim.p,
+                fcw::launch(bfs_claim_kernel, dim3(blocks_for(fn, bs)), dim3(bs), 0, st, order.p, f0, fn, uoff.p, cols.p, visited.p, claim.p);
+                fcw::launch(bfs_count_kernel, dim3(blocks_for(fn, bs)), dim3(bs), 0, st, order.p, f0, fn, uoff.p, cols.p, visited.p, claim.p,
                                                                     cnt.p);
                 const int nn = scan(st, cnt.p, fn, off.p, total.p, sums.p);
                 if (nn > 0) {
-                    bfs_emit_kernel<<<blocks_for(fn, bs), bs, 0, st>>>(order.p, f0, fn, uoff.p, cols.p, visited.p,
+                    fcw::launch(bfs_emit_kernel, dim3(blocks_for(fn, bs)), dim3(bs), 0, st, order.p, f0, fn, uoff.p, cols.p, visited.p,
                                                                        claim.p, off.p);
-                    bfs_mark_kernel<<<blocks_for(nn, bs), bs, 0, st>>>(order.p, f0 + fn, nn, visited.p, claim.p);
+                    fcw::launch(bfs_mark_kernel, dim3(blocks_for(nn, bs)), dim3(bs), 0, st, order.p, f0 + fn, nn, visited.p, claim.p);
                 }
                 f0 += fn;
                 fn = nn;
@@ -411,7 +411,7 @@ int mesh_prep_device(int n, const double* Vh, int m, const int* Fh, const unsign
             if (placed >= ns) break;
             int h_best = ns;
             check(hipMemcpyAsync(best.p, &h_best, sizeof(int), hipMemcpyHostToDevice, st), "best");
-            first_unvisited_kernel<<<blocks_for(ns, bs), bs, 0, st>>>(ns, visited.p, best.p);
+            fcw::launch(first_unvisited_kernel, dim3(blocks_for(ns, bs)), dim3(bs), 0, st, ns, visited.p, best.p);
             check(hipMemcpyAsync(&h_best, best.p, sizeof(int), hipMemcpyDeviceToHost, st), "best back");
             check(hipStreamSynchronize(st), "components");
             if (h_best >= ns) throw std::string("breadth-first order lost vertices");

@@ -20,7 +20,9 @@
 // the device with a one-wave probe).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "fcw_device.h"
 
@@ -29,6 +31,19 @@ namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef short bf4 __attribute__((ext_vector_type(4)));  // four bf16 values
+
+// Split-BF16 form of an FP32 value: hi = x rounded to bf16 (nearest even), lo = the remainder
+// x - hi rounded to bf16; hi + lo carries 16 significant bits. Packed as (lo << 16) | hi.
+__device__ __forceinline__ unsigned bf16_rne(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ unsigned split_bf16(float x) {
+    const unsigned hi = bf16_rne(x);
+    const float rest = x - __uint_as_float(hi << 16);
+    return (bf16_rne(rest) << 16) | hi;
+}
 
 // One 16 x 16 x 4 matrix-core step per value type, and where it leaves row r of lane group q
 // (checked on the device with one-wave probes): FP64 D rows q + 4 r, FP32 D rows 4 q + r.
@@ -76,14 +91,23 @@ enum class Sweep { Forward, Backward };
 // unsplit product only in rounding.
 constexpr int kXcds = 8;
 
-template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS, bool DT = false, int SPLIT = 0>
+// PT = uint32_t (with T = float): split-BF16 panels, each entry packed hi | lo << 16; the right-hand
+// side is split the same way while it is staged, and each 16 x 16 block of a depth step takes three
+// v_mfma_f32_16x16x16bf16_1k products (hi*hi + hi*lo + lo*hi, FP32 accumulation; the lo*lo term,
+// 2^-16 of the product, is dropped). Operand layout (checked on the device with a one-wave probe):
+// A[i][4q + v] in lane i + 16q, element v; B[4q + v][j] in lane j + 16q; D rows 4q + r as for FP32.
+constexpr int kBfPad = kTileK + 4;  // bf16 LDS row: the 16 depth values of one tile row / column
+
+template <typename T, Sweep MODE, bool IDX_LDS, bool KIDS, bool DT = false, int SPLIT = 0, typename PT = T>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(
     int ntiles, const LevelTile* __restrict__ tiles, int ldk, const int* __restrict__ node_start, const int* __restrict__ node_s,
     const int* __restrict__ node_b, const int64_t* __restrict__ node_boff, const int* __restrict__ bidx,
     const int64_t* __restrict__ node_poff, const int64_t* __restrict__ node_uoff, const int* __restrict__ cown,
-    const int* __restrict__ ctail, const T* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
+    const int* __restrict__ ctail, const PT* __restrict__ panel, const T* __restrict__ Bsrc, const T* Zsrc, T* Out,
     T* U, double* acc64, int acc_mode, double* __restrict__ Dt, int ldd, int zrow, int nsplit, double* __restrict__ part) {
     static_assert(SPLIT == 0 || sizeof(T) == sizeof(double), "split depth is FP64 only");
+    constexpr bool BF = std::is_same<PT, uint32_t>::value;
+    static_assert(!BF || std::is_same<T, float>::value, "split-BF16 panels go with FP32 right-hand sides");
     const int ncol = ldk / kTileN;
     const int xcd = blockIdx.x % kXcds;
     const int ps = SPLIT == 1 ? (int)(blockIdx.x / kXcds) % nsplit : 0;  // this workgroup's depth part
@@ -93,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     if (tile >= ntiles) return;
     const LevelTile t = tiles[tile];
     const int s = node_s[t.node], b = node_b[t.node], f = s + b, start = node_start[t.node];
-    const T* __restrict__ P = panel + node_poff[t.node];
+    const PT* __restrict__ P = panel + node_poff[t.node];
     const int* __restrict__ B = (MODE == Sweep::Backward) ? bidx + node_boff[t.node] : nullptr;
     const int n0 = (j - (j / ncol) * ncol) * kTileN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -116,6 +140,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     T(&As)[kTileK][kTileM + kPad] = sbuf[0];
     T(&Bs)[kTileK][kTileN + kPad] = sbuf[1];
     __shared__ int sidx[2 * kIdxCap];
+    // split-BF16 stages, depth contiguous: [0] A hi, [1] A lo (tile row, depth), [2] B hi, [3] B lo (column, depth)
+    __shared__ __align__(16) unsigned short bfs[BF ? 4 : 1][BF ? kTileM : 1][BF ? kBfPad : 1];
     // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
     // contributor rows for each of the node's s own rows.
     if (IDX_LDS && SPLIT != 2) {
@@ -146,7 +172,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     // address. fetch() only issues the loads of the next step (and records which elements are
     // kept, subtracted or negated); stash() applies that after the current step's MFMAs, so no
     // arithmetic on a loaded value forces a wait before the matrix cores have their work.
-    T ra[kLoadsA], rb[kLoadsB], ru[kLoadsB], rv[kLoadsB];
+    PT ra[kLoadsA];
+    T rb[kLoadsB], ru[kLoadsB], rv[kLoadsB];
     unsigned keep_a = 0, keep_b = 0, bit_u = 0, bit_v = 0;
     auto fetch = [&](int k0) {
         keep_a = keep_b = bit_u = bit_v = 0;
@@ -201,6 +228,32 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         }
     };
     auto stash = [&]() {
+        if constexpr (BF) {
+#pragma unroll
+            for (int i = 0; i < kLoadsA; ++i) {
+                const int e = tid + i * kThreads;
+                const unsigned v = ((keep_a >> i) & 1u) ? ra[i] : 0u;
+                const int m = (MODE == Sweep::Forward) ? e % kTileM : e / kTileK;
+                const int k = (MODE == Sweep::Forward) ? e / kTileM : e % kTileK;
+                bfs[0][m][k] = (unsigned short)(v & 0xffffu);
+                bfs[1][m][k] = (unsigned short)(v >> 16);
+            }
+#pragma unroll
+            for (int i = 0; i < kLoadsB; ++i) {
+                const int e = tid + i * kThreads;
+                float v;
+                if (MODE == Sweep::Forward && KIDS)
+                    v = rb[i] - ru[i] - rv[i];
+                else if (MODE == Sweep::Forward)
+                    v = rb[i];
+                else
+                    v = ((bit_u >> i) & 1u) ? -rb[i] : rb[i];
+                const unsigned sv = ((keep_b >> i) & 1u) ? split_bf16(v) : 0u;
+                bfs[2][e % kTileN][e / kTileN] = (unsigned short)(sv & 0xffffu);
+                bfs[3][e % kTileN][e / kTileN] = (unsigned short)(sv >> 16);
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < kLoadsA; ++i) {
             const int e = tid + i * kThreads;
@@ -246,6 +299,27 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         stash();
         __syncthreads();
         if (k0 + kTileK < kend) fetch(k0 + kTileK);
+        if constexpr (BF) {
+            bf4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                ah[q] = *reinterpret_cast<const bf4*>(&bfs[0][wm0 + 16 * q + lrow][4 * lk]);
+                al[q] = *reinterpret_cast<const bf4*>(&bfs[1][wm0 + 16 * q + lrow][4 * lk]);
+                bh[q] = *reinterpret_cast<const bf4*>(&bfs[2][wn0 + 16 * q + lrow][4 * lk]);
+                bl[q] = *reinterpret_cast<const bf4*>(&bfs[3][wn0 + 16 * q + lrow][4 * lk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    flt4 c = acc[i][j];
+                    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bh[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah[i], bl[j], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al[i], bh[j], c, 0, 0, 0);
+                    acc[i][j] = c;
+                }
+            continue;
+        }
 #pragma unroll
         for (int k4 = 0; k4 < kTileK; k4 += 4) {
             const T a0 = As[k4 + lk][wm0 + lrow];
@@ -376,19 +450,19 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     }
 }
 
-template <typename T>
+template <typename T, typename PT = T>
 void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
               const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
-              const int* contrib_tail, const T* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves,
+              const int* contrib_tail, const PT* panel, const T* X, T* Y, T* U, bool idx_lds, bool leaves,
               int zrow, int nsplit, double* part) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));  // 8 runs of ceil(ntiles / 8)
     const dim3 grid_split(grid.x * (nsplit > 1 ? nsplit : 1));
 #define FCW_FWD(LDS, KIDS, SPL, G)                                                                            \
-    sweep_kernel<T, Sweep::Forward, LDS, KIDS, false, SPL><<<G, kThreads, 0, st>>>(                           \
+    fcw::launch(sweep_kernel<T, Sweep::Forward, LDS, KIDS, false, SPL, PT>, dim3(G), dim3(kThreads), 0, st,                        \
         ntiles, tiles, ldk, node_start, node_s, node_b, nullptr, nullptr, node_poff, node_uoff, contrib_own, contrib_tail, \
         panel, X, nullptr, Y, U, nullptr, 0, nullptr, 0, zrow, nsplit, part)
-    if constexpr (sizeof(T) == sizeof(double)) {
+    if constexpr (sizeof(T) == sizeof(double) && std::is_same<T, PT>::value) {
         if (nsplit > 1 && !leaves) {
             if (idx_lds) {
                 FCW_FWD(true, true, 1, grid_split);
@@ -409,16 +483,16 @@ void fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 #undef FCW_FWD
 }
 
-template <typename T>
+template <typename T, typename PT = T>
 void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
-              const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const T* panel,
+              const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff, const PT* panel,
               const T* Y, T* Z, bool idx_lds, double* acc64, int acc_mode, double* Dt, int ldd, int nsplit,
               double* part) {
     if (ntiles <= 0) return;
     const dim3 grid((ntiles + kXcds - 1) / kXcds * kXcds * (ldk / kTileN));
     const dim3 grid_split(grid.x * (nsplit > 1 ? nsplit : 1));
 #define FCW_BWD(LDS, DTW, SPL, G)                                                                               \
-    sweep_kernel<T, Sweep::Backward, LDS, true, DTW, SPL><<<G, kThreads, 0, st>>>(                             \
+    fcw::launch(sweep_kernel<T, Sweep::Backward, LDS, true, DTW, SPL, PT>, dim3(G), dim3(kThreads), 0, st,                          \
         ntiles, tiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, nullptr, nullptr, nullptr, panel, Y, Z, \
         Z, nullptr, acc64, acc_mode, Dt, ldd, 0, nsplit, part)
 #define FCW_BWD_ALL(SPL, G)          \
@@ -430,7 +504,7 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
         FCW_BWD(false, true, SPL, G);\
     else                             \
         FCW_BWD(false, false, SPL, G)
-    if constexpr (sizeof(T) == sizeof(double)) {
+    if constexpr (sizeof(T) == sizeof(double) && std::is_same<T, PT>::value) {
         if (nsplit > 1) {
             FCW_BWD_ALL(1, grid_split);
             FCW_BWD_ALL(2, grid);
@@ -442,7 +516,38 @@ void bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const
 #undef FCW_BWD
 }
 
+__global__ void split_bf16_kernel(const double* __restrict__ src, uint32_t* __restrict__ dst, int64_t count) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const double x = src[i];
+        const unsigned hi = bf16_rne((float)x);
+        const double rest = x - (double)__uint_as_float(hi << 16);
+        dst[i] = (bf16_rne((float)rest) << 16) | hi;
+    }
+}
+
 }  // namespace
+
+void launch_to_bf16x2(hipStream_t st, const double* src, uint32_t* dst, int64_t count) {
+    if (count <= 0) return;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 65536);
+    fcw::launch(split_bf16_kernel, dim3(dim3((unsigned)blocks)), dim3(256), 0, st, src, dst, count);
+}
+
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+                     const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
+                     const int* contrib_tail, const uint32_t* panel, const float* X, float* Y, float* U, bool idx_lds,
+                     bool leaves, int zrow) {
+    fwd_gemm<float, uint32_t>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_poff, node_uoff, contrib_own,
+                              contrib_tail, panel, X, Y, U, idx_lds, leaves, zrow, 1, nullptr);
+}
+
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+                     const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff,
+                     const uint32_t* panel, const float* Y, float* Z, bool idx_lds, double* acc64, int acc_mode,
+                     double* Dt, int ldd) {
+    bwd_gemm<float, uint32_t>(st, tiles, ntiles, ldk, node_start, node_s, node_b, node_boff, bidx, node_poff, panel, Y, Z,
+                              idx_lds, acc64, acc_mode, Dt, ldd, 1, nullptr);
+}
 
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,

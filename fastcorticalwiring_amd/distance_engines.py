@@ -57,8 +57,8 @@ def _backend_mixin(base, potpourri=False):
         engine_kwargs:
           t_coef (float, default 1.0): t = t_coef * mean_edge**2 (fastcw_lean.py:485-489), or
           diffusion_length_mm (float): t = L**2 (the potpourri convention, distance_engines.py:119-124);
-          poisson_shift (1e-10), block (256 sources per solve), precision ('fp64' | 'fp32_ir'),
-          ir_steps (2, refinement steps of 'fp32_ir'), leaf_size (64), device (0), host_threads (0),
+          poisson_shift (1e-10), block (256 sources per solve), precision ('fp64' | 'fp32_ir' | 'bf16x3_ir'),
+          ir_steps (2, refinement steps of the refined forms), leaf_size (64), device (0), host_threads (0),
           streams (1: this adapter solves one block per miss, so more workspaces would sit idle),
           use_robust (False: operators on the surface's own triangulation, as ConsistentBatchHeatEngine;
           True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode),
@@ -116,8 +116,8 @@ def _backend_mixin(base, potpourri=False):
             if not np.isfinite(shift) or shift <= 0.0:
                 raise ValueError("poisson_shift must be finite and positive.")
             precision = str(kw.pop("precision", "fp64")).lower()
-            if precision not in ("fp64", "fp32_ir"):
-                raise ValueError("precision must be 'fp64' or 'fp32_ir'.")
+            if precision not in ("fp64", "fp32_ir", "bf16x3_ir"):
+                raise ValueError("precision must be 'fp64', 'fp32_ir' or 'bf16x3_ir'.")
             self.use_robust = robust
             try:
                 from . import _lib

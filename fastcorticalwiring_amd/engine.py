@@ -39,7 +39,7 @@ class HeatContext:
         opt.t = float(t) if t is not None else 0.0
         opt.t_coef = float(t_coef)
         opt.poisson_shift = float(poisson_shift)
-        opt.precision = {"fp64": _lib.FCW_FP64, "fp32_ir": _lib.FCW_FP32_IR}[precision]
+        opt.precision = {"fp64": _lib.FCW_FP64, "fp32_ir": _lib.FCW_FP32_IR, "bf16x3_ir": _lib.FCW_BF16X3_IR}[precision]
         opt.block = int(block)
         opt.leaf_size = int(leaf_size)
         opt.host_threads = int(host_threads)

@@ -24,9 +24,12 @@ typedef struct fcw_ctx fcw_ctx;
 /* Arithmetic of the two triangular sweeps per system. */
 enum {
     FCW_FP64 = 0,    /* plain FP64 panels and RHS blocks */
-    FCW_FP32_IR = 1  /* Poisson system: FP32 panel products with FP32 accumulation, refined against the FP64
+    FCW_FP32_IR = 1, /* Poisson system: FP32 panel products with FP32 accumulation, refined against the FP64
                         residual; the heat system stays FP64 (its solution spans 1 to ~1e-141 at the
                         164k-vertex mesh, below FP32's range) */
+    FCW_BF16X3_IR = 2 /* as FCW_FP32_IR with split-BF16 panel products: panel and right-hand side each split
+                        into bf16 hi + lo, three bf16 matrix-core products (hi*hi + hi*lo + lo*hi) with FP32
+                        accumulation, refined against the FP64 residual */
 };
 
 /* Scalar outputs per source, in this order (fcw_metrics_out.scalars is k x FCW_NSCALARS). */
@@ -47,12 +50,12 @@ typedef struct {
     double t;              /* diffusion time; <= 0 selects t_coef * mean_edge^2 */
     double t_coef;         /* default 1.0 (fastcw_lean.py:481, 485-489) */
     double poisson_shift;  /* default 1e-10: Poisson matrix L + shift*M (fastcw_lean.py:500) */
-    int precision;         /* FCW_FP64 or FCW_FP32_IR */
+    int precision;         /* FCW_FP64, FCW_FP32_IR or FCW_BF16X3_IR */
     int block;             /* maximum sources per RHS block (padded to a multiple of 64); default 256 */
     int leaf_size;         /* nested-dissection leaf size; default 64 */
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
-    int ir_steps;          /* FCW_FP32_IR: residual + correction solves after the first FP32 solve; default 2 (one
+    int ir_steps;          /* refined forms: residual + correction solves after the first solve; default 2 (one
                               step leaves ~2e-7 of the largest distance, 6e-5 relative on the smallest
                               distances to the boundary of the 164k-vertex surface) */
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces), 1..16;
@@ -65,9 +68,10 @@ typedef struct {
                               on when streams <= 2 */
     int band_cap;          /* band faces per scale held per source by the metric kernel (default 6144); a band
                               that does not fit is evaluated from every face instead (same result, slower) */
-    int graphs;            /* 1 (default): each stream captures the launches that are the same for every full
+    int graphs;            /* 1 (default): each stream records the launches that are the same for every full
                               block (the heat backward sweep; divergence + Poisson solve) into HIP graphs once
-                              and replays them; 0: every launch issued one by one */
+                              (built node by node, no stream capture) and replays them; 0: every launch issued
+                              one by one */
 } fcw_options;
 
 typedef struct {

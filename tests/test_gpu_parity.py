@@ -315,3 +315,25 @@ def test_graph_replays_match_eager_launches(c2, precision):
                 assert np.max(rel_err(w1[key][fin], w0[key][fin])) < 1e-12, (rep, key)
             ctx.distances(src[:37])  # other eager work on the device between the replays
     print(f"{precision}: 3 rounds of graph replays equal the eager launches", flush=True)
+
+
+def test_bf16x3_refined_poisson_converges_to_the_oracle(c2):
+    """FCW_BF16X3_IR: the Poisson sweeps as three split-BF16 matrix-core products (hi*hi + hi*lo +
+    lo*hi, FP32 accumulation), refined against the FP64 residual. Each product keeps about 16
+    significant bits, so the unrefined solve is far coarser than FP32's; the refinement steps
+    contract the error until the distances meet 1e-5 relative of the FP64 oracle (d > 5h), and
+    with enough steps the metrics do too."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    s, mesh, ctx, H = c2
+    src = np.arange(3, mesh.n, 41)[:128]
+    Dref = H.compute_distance_batch(src)
+    far = Dref > 5 * mesh.health_h
+    errs = {}
+    for steps in (0, 2, 4, 6):
+        with HeatContext(mesh.V, mesh.F, block=128, precision="bf16x3_ir", ir_steps=steps) as c:
+            D = c.distances(src)
+        assert np.all(D[np.arange(src.size), src] == 0.0)
+        errs[steps] = float(np.max(rel_err(D[far], Dref[far])))
+        print(f"bf16x3 ir_steps {steps}: rel(d>5h) {errs[steps]:.2e}", flush=True)
+    assert errs[6] < 1e-5 and errs[6] <= errs[2] <= errs[0]
