@@ -8,8 +8,8 @@ samples between scales. One step = one RHS block of `--block` sources through
 heat solve -> fused divergence -> Poisson solve -> metric kernel, with outputs
 copied back. The mesh, operators and factors are resident in HBM before the
 timed region; the default run times every cortical source once, in each
-precision form (plain FP64, and FP32 Poisson sweeps refined against the FP64
-residual).
+precision form (plain FP64; FP32 Poisson sweeps refined against the FP64
+residual; split-BF16x3 Poisson sweeps refined the same way).
 
 Also reported:
 * the end-to-end latency of one whole hemisphere, measured (surface in host
@@ -355,7 +355,7 @@ def main():
     ap.add_argument("--autotune", default="on", choices=("on", "off"),
                     help="time block widths {128,256,512} x leaf sizes {32,64,96,128} first and use the fastest")
     ap.add_argument("--streams", type=int, default=8, help="RHS blocks in flight (one HIP stream each)")
-    ap.add_argument("--precision", default="auto", help="fp64 | fp32_ir | auto (every form)")
+    ap.add_argument("--precision", default="auto", help="fp64 | fp32_ir | bf16x3_ir | auto (every form)")
     ap.add_argument("--cpu-baseline", default="on", choices=("on", "off"))
     ap.add_argument("--cpu-sources", type=int, default=512)
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU processes (0: host share minus one)")
@@ -391,7 +391,7 @@ def main():
 
     if args.calibrate_bytes:
         _lib.check(_lib.lib().fcw_calibration_copy(int(args.calibrate_bytes), 0))
-    forms = ["fp64", "fp32_ir"] if args.precision == "auto" else [args.precision]
+    forms = ["fp64", "fp32_ir", "bf16x3_ir"] if args.precision == "auto" else [args.precision]
     tuned = None
     block, leaf = args.block, args.leaf_size
     if args.autotune == "on":
@@ -485,7 +485,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best["precision"] == "fp64" else "f32+f64ir",
+        "dtype": {"fp64": "f64", "fp32_ir": "f32+f64ir", "bf16x3_ir": "bf16x3+f64ir"}[best["precision"]],
         "data": "synthetic",
         "config": {
             "workload": WORKLOAD_NAMES.get(args.workload, str(args.workload)),

@@ -74,7 +74,7 @@ class WiringAnalysis:
         self.ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t,
                                poisson_shift=float(kw.pop("poisson_shift", 1e-10)), block=int(kw.pop("block", 256)),
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
-                               ir_steps=int(kw.pop("ir_steps", 2)), device=int(kw.pop("device", 0)),
+                               ir_steps=kw.pop("ir_steps", None), device=int(kw.pop("device", 0)),
                                host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)),
                                robust=self.use_robust, split_depth=int(kw.pop("split_depth", 0)))
         if kw:

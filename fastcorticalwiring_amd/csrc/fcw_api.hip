@@ -951,7 +951,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->leaf_size = 64;
     opt->host_threads = 0;
     opt->device = 0;
-    opt->ir_steps = 2;
+    opt->ir_steps = -1;
     opt->streams = 8;
     opt->robust = 0;
     opt->sparse_rhs = 1;
@@ -984,7 +984,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         if (opt_in) opt = *opt_in;
         if (opt.precision != FCW_FP64 && opt.precision != FCW_FP32_IR && opt.precision != FCW_BF16X3_IR)
             throw Fail("unknown precision");
-        if (opt.ir_steps < 0 || opt.ir_steps > 8) throw Fail("ir_steps must be in 0..8");
+        if (opt.ir_steps < 0) opt.ir_steps = opt.precision == FCW_BF16X3_IR ? 6 : 2;
+        if (opt.ir_steps > 8) throw Fail("ir_steps must be in 0..8 (or negative for the form's default)");
         if (opt.block <= 0) opt.block = 256;
         if (opt.streams <= 0) opt.streams = 1;
         if (opt.streams > 16) throw Fail("streams must be in 1..16");

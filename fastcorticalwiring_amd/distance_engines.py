@@ -58,7 +58,7 @@ def _backend_mixin(base, potpourri=False):
           t_coef (float, default 1.0): t = t_coef * mean_edge**2 (fastcw_lean.py:485-489), or
           diffusion_length_mm (float): t = L**2 (the potpourri convention, distance_engines.py:119-124);
           poisson_shift (1e-10), block (256 sources per solve), precision ('fp64' | 'fp32_ir' | 'bf16x3_ir'),
-          ir_steps (2, refinement steps of the refined forms), leaf_size (64), device (0), host_threads (0),
+          ir_steps (refinement steps of the refined forms: 2 for 'fp32_ir', 6 for 'bf16x3_ir'), leaf_size (64), device (0), host_threads (0),
           streams (1: this adapter solves one block per miss, so more workspaces would sit idle),
           use_robust (False: operators on the surface's own triangulation, as ConsistentBatchHeatEngine;
           True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode),
@@ -132,7 +132,7 @@ def _backend_mixin(base, potpourri=False):
             self.backend_info = info
             self._ctx = HeatContext(self.vertices, self.faces, t_coef=self.t_coef, t=t, poisson_shift=shift,
                                     block=int(kw.pop("block", 256)), leaf_size=int(kw.pop("leaf_size", 64)),
-                                    precision=precision, ir_steps=int(kw.pop("ir_steps", 2)),
+                                    precision=precision, ir_steps=kw.pop("ir_steps", None),
                                     device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)),
                                     streams=int(kw.pop("streams", 1)), robust=robust,
                                     split_depth=int(kw.pop("split_depth", 0)))

@@ -25,7 +25,7 @@ class HeatContext:
     """
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
-                 precision="fp64", ir_steps=2, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False,
+                 precision="fp64", ir_steps=None, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False,
                  split_depth=0, band_cap=0, graphs=True):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
@@ -44,7 +44,7 @@ class HeatContext:
         opt.leaf_size = int(leaf_size)
         opt.host_threads = int(host_threads)
         opt.device = int(device)
-        opt.ir_steps = int(ir_steps)
+        opt.ir_steps = -1 if ir_steps is None else int(ir_steps)
         opt.streams = int(streams)
         opt.sparse_rhs = 1 if sparse_rhs else 0
         opt.robust = 1 if robust else 0

@@ -55,9 +55,10 @@ typedef struct {
     int leaf_size;         /* nested-dissection leaf size; default 64 */
     int host_threads;      /* threads for the host factorisation; 0 = OpenMP default */
     int device;            /* HIP device ordinal */
-    int ir_steps;          /* refined forms: residual + correction solves after the first solve; default 2 (one
-                              step leaves ~2e-7 of the largest distance, 6e-5 relative on the smallest
-                              distances to the boundary of the 164k-vertex surface) */
+    int ir_steps;          /* refined forms: residual + correction solves after the first solve; < 0 (default):
+                              2 for FCW_FP32_IR (one step leaves ~2e-7 of the largest distance, 6e-5 relative
+                              on the smallest distances to the boundary of the 164k-vertex surface), 6 for
+                              FCW_BF16X3_IR (4 leave 4.5e-5 relative there) */
     int streams;           /* RHS blocks in flight on separate HIP streams (own workspaces), 1..16;
                               default 8 (run concurrently when GPU_MAX_HW_QUEUES >= streams) */
     int sparse_rhs;        /* 1: the heat forward sweep visits only supernodes on source-to-root paths */

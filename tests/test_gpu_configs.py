@@ -12,7 +12,8 @@ Tolerances:
   relative where d_ref > 5h as compare_fastcw_lean.py:235-239); radii,
   perimeters and supplementary samples within 1e-8; MSD and distance to boundary
   within 1e-6 (the fixture holds them as float32, as the reference's arrays do).
-* FP32+IR form: every output within 1e-5 relative, the hot path's target.
+* FP32+IR and split-BF16x3+IR forms: every output within 1e-5 relative, the hot
+  path's target.
 * Flags and cleaned-sample counts are equal; sample values within 1e-5.
 """
 
@@ -32,6 +33,7 @@ pytestmark = pytest.mark.gpu
 TOL = {
     "fp64": dict(dist=1e-9, geo=1e-8, scalar=1e-6, sample=1e-5),
     "fp32_ir": dict(dist=1e-5, geo=1e-5, scalar=1e-5, sample=1e-5),
+    "bf16x3_ir": dict(dist=1e-5, geo=1e-5, scalar=1e-5, sample=1e-5),
 }
 
 _cache = {}
@@ -96,7 +98,7 @@ def check_outputs(sc, radii, perim, extra_r, extra_a, sample_r, sample_a, fx, to
     return worst
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32_ir"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir", "bf16x3_ir"])
 @pytest.mark.parametrize("idx", [1, 2, 3])
 def test_folded_workload_matches_oracle(idx, precision):
     from fastcorticalwiring_amd.engine import HeatContext
@@ -117,21 +119,22 @@ def test_folded_workload_matches_oracle(idx, precision):
     print("  worst relative:", {k: f"{v:.1e}" for k, v in worst.items()}, flush=True)
 
 
-def test_fp32_ir_matches_fp64_on_every_source_of_the_largest_surface():
-    """FP32+IR against the FP64 form (itself pinned to the oracle above) over all 148,165 cortical
-    sources of the 163,842-vertex surface: every metric within 1e-5 relative."""
+@pytest.mark.parametrize("form", ["fp32_ir", "bf16x3_ir"])
+def test_refined_forms_match_fp64_on_every_source_of_the_largest_surface(form):
+    """The refined forms against the FP64 form (itself pinned to the oracle above) over all 148,165
+    cortical sources of the 163,842-vertex surface: every metric within 1e-5 relative."""
     from fastcorticalwiring_amd.engine import HeatContext
     from fastcorticalwiring_amd import mesh_prep
 
     fx, mesh, _ = folded(3)
     order = np.asarray(mesh.bfs_order, dtype=np.int32)
     out = {}
-    for precision in ("fp64", "fp32_ir"):
+    for precision in ("fp64", form):
         print("all sources,", precision, flush=True)
         with HeatContext(mesh.V, mesh.F, block=256, precision=precision) as ctx:
             set_metrics(ctx, fx, mesh)
             out[precision] = ctx.wiring(order)
-    a, b = out["fp32_ir"], out["fp64"]
+    a, b = out[form], out["fp64"]
     assert np.array_equal(a["scalars"][:, 0], b["scalars"][:, 0])
     ok = b["scalars"][:, 0] == 0
     assert ok.sum() == order.size
@@ -143,7 +146,7 @@ def test_fp32_ir_matches_fp64_on_every_source_of_the_largest_surface():
         worst[key] = float(np.max(rel_err(x[fin], y[fin])))
     for col, key in ((3, "dist_to_boundary"), (4, "msd_u"), (5, "msd_w")):
         worst[key] = float(np.max(rel_err(a["scalars"][ok, col], b["scalars"][ok, col])))
-    print("fp32_ir vs fp64, all sources:", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    print(form, "vs fp64, all sources:", {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
     assert max(worst.values()) < 1e-5, worst
 
 
