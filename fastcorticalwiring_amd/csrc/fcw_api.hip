@@ -94,7 +94,7 @@ enum TimedKernel {
     kFwd32 = 3,    // FP32 forward sweep launch (refined solve)
     kBwd32 = 4,    // FP32 backward sweep launch
     kDivergence = 5,
-    kChunkStats = 6,
+    kStatsKernel = 6,  // chunk statistics
     kResidual = 7,  // FP64 residual of the refined solve
     kNumTimed = 8
 };
@@ -181,7 +181,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     DBuf<double> X, Yh, Yp, Z, Uh, Up, Dt, band;
     DBuf<double> part;  // partial products of the split-depth sweep launches
-    DBuf<double> cpart;              // chunk_stats records (k x nch x kChunkStats)
+    DBuf<double> cpart;              // chunk_stats records (ldk x nch x kChunkStats, at chunk_rec_index)
     DBuf<float> X32, Y32, U32, Z32;  // refined solves: FP32 right-hand side and sweeps
     DBuf<float> ch_lo, ch_hi;        // chunk distance ranges (k x nch)
     DBuf<int> src_pos;
@@ -748,7 +748,7 @@ struct fcw_ctx {
                 HIPCHK(hipEventCreate(&mb));
                 HIPCHK(hipEventRecord(ma, w.stream));
             }
-            EventPair* ec = ev_begin(w, kChunkStats);
+            EventPair* ec = ev_begin(w, kStatsKernel);
             fcw::launch_chunk_stats(w.stream, k, ldk, nch, w.Z.p, w.src_pos.p, V.p, vw.p, isb.p, cvptr.p, cverts.p,
                                     cnown.p, eps, health_h, w.ch_lo.p, w.ch_hi.p, w.cpart.p);
             ev_end(w, ec);

@@ -161,9 +161,9 @@ struct MetricsDev {
     // chunk bounds): per source and 256-face chunk, the distance range of the chunk's vertices
     // and kChunkStats partial sums over the vertices the chunk owns.
     int nch;
-    const float* ch_lo;        // k x nch, rounded down
-    const float* ch_hi;        // k x nch, rounded up
-    const double* part;        // k x kChunkStats x nch
+    const float* ch_lo;        // rounded down, at chunk_range_index(j, c)
+    const float* ch_hi;        // rounded up, at chunk_range_index(j, c)
+    const double* part;        // at chunk_rec_index(j, c, q)
     const double* chunk_area;  // nch: area of the chunk's faces
     // Fields are stored unshifted when shift_src is set: source j's distances are
     // Dt[j][p] - shift_src[src_pos[j] * shift_ld + j] (its solution at its own position).
@@ -174,6 +174,20 @@ struct MetricsDev {
 // Partial record of chunk_stats: sum d (d > eps), sum d*w, sum w, max finite d, min finite d on the
 // boundary, then as exact counts: d < -h, chord violations, d > eps, non-finite, finite boundary.
 constexpr int kChunkStats = 10;
+
+// Where chunk_stats puts source j's record q of chunk c (and its distance range). Sources are grouped
+// by eight with the same j mod 8 within their 64-source column slice: a wave of 64 consecutive
+// sources then writes whole 64-byte lines (eight groups of eight doubles), and the eight sources of a
+// group are metric workgroups that run on one XCD (workgroup j goes to XCD j mod 8), so they share
+// those lines in its L2. Arrays hold ldk x nch x kChunkStats records (ldk x nch ranges).
+__host__ __device__ inline int64_t chunk_rec_index(int j, int c, int q, int nch) {
+    const int64_t g = (int64_t)(j >> 6) * 8 + (j & 7);
+    return ((g * nch + c) * kChunkStats + q) * 8 + ((j >> 3) & 7);
+}
+__host__ __device__ inline int64_t chunk_range_index(int j, int c, int nch) {
+    const int64_t g = (int64_t)(j >> 6) * 8 + (j & 7);
+    return (g * nch + c) * 8 + ((j >> 3) & 7);
+}
 
 struct MetricsOut {
     double* scal;      // k x kSlotCount

@@ -355,19 +355,18 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const double h = md.health_h;
     if (md.part) {
         // the vertex pass was made for the whole block by chunk_stats: add up its chunk records
-        const double* pr = md.part + (int64_t)j * md.nch * kChunkStats;  // kChunkStats arrays of nch
-        const int64_t st = md.nch;
         for (int c = tid; c < md.nch; c += NT) {
-            v4[0] += pr[c];
-            v4[1] += pr[st + c];
-            v4[2] += pr[2 * st + c];
-            v4[3] = fmax(v4[3], pr[3 * st + c]);
-            b4[0] = fmin(b4[0], pr[4 * st + c]);
-            i4[0] += (int)pr[5 * st + c];
-            i4[1] += (int)pr[6 * st + c];
-            i4[2] += (int)pr[7 * st + c];
-            i4[3] += (int)pr[8 * st + c];
-            bi4[0] += (int)pr[9 * st + c];
+            const double* pr = md.part + chunk_rec_index(j, c, 0, md.nch);  // record values 8 apart
+            v4[0] += pr[0];
+            v4[1] += pr[8];
+            v4[2] += pr[16];
+            v4[3] = fmax(v4[3], pr[24]);
+            b4[0] = fmin(b4[0], pr[32]);
+            i4[0] += (int)pr[40];
+            i4[1] += (int)pr[48];
+            i4[2] += (int)pr[56];
+            i4[3] += (int)pr[64];
+            bi4[0] += (int)pr[72];
         }
     }
     for (int p0 = md.part ? n : tid; p0 < n; p0 += NT * UNR) {
@@ -477,16 +476,16 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     const int nbins = s_nbins;
     // Chunk distance ranges: from chunk_stats when given (global), else recorded below (LDS).
     const bool have_ch = md.ch_lo != nullptr;
-    const float* gch_lo = have_ch ? md.ch_lo + (int64_t)j * md.nch : nullptr;
-    const float* gch_hi = have_ch ? md.ch_hi + (int64_t)j * md.nch : nullptr;
+    auto gch_lo = [&](int c) { return md.ch_lo[chunk_range_index(j, c, md.nch)]; };
+    auto gch_hi = [&](int c) { return md.ch_hi[chunk_range_index(j, c, md.nch)]; };
     const bool chunked = have_ch || m <= kMaxChunks * kWave * UNR;
     // The face passes test every chunk's range before loading its faces, one chunk per wave step:
     // from LDS that test costs a few cycles, from global memory a full load latency per step.
     const bool ch_in_lds = !have_ch || md.nch <= kMaxChunks;
     if (have_ch && ch_in_lds)
         for (int c = tid; c < md.nch; c += NT) {
-            ch_lo[c] = gch_lo[c];
-            ch_hi[c] = gch_hi[c];
+            ch_lo[c] = gch_lo(c);
+            ch_hi[c] = gch_hi(c);
         }
     // With chunk ranges and a finite field, only faces that start below `cut` are histogrammed:
     // the chunks entirely below bin K already hold the largest target's area, so every radius,
@@ -500,7 +499,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                 break;
             }
         for (int c = tid; c < md.nch; c += NT) {
-            const double hi = (double)gch_hi[c];
+            const double hi = (double)gch_hi(c);
             if (hi > -INFINITY) atomicAdd(&Hmin[bin_of(hi, wb, nbins)], md.chunk_area[c]);
         }
         __syncthreads();
@@ -568,7 +567,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         }
     } else {
         for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
-            if (have_ch && (double)gch_lo[base / (kWave * UNR)] >= cut) continue;  // beyond every band
+            if (have_ch && (double)gch_lo(base / (kWave * UNR)) >= cut) continue;  // beyond every band
             FaceBatch fb;
             load_faces<true>(faces, mesh.larea, d, sh, m, base, lane, fb);
             double cmin, cmax;
@@ -813,8 +812,8 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
         }
     };
     auto meets_bands = [&](int c) {
-        const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo[c];
-        const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi[c];
+        const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo(c);
+        const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi(c);
         return !(chi < skip_lo || clo >= skip_hi);
     };
     const int nchunks = (m + kWave * UNR - 1) / (kWave * UNR);
@@ -1155,11 +1154,11 @@ __global__ __launch_bounds__(256) void chunk_stats_kernel(int ldk, int k, int nc
     if (jl >= k) return;
     const double r[kChunkStats] = {sd, sdw, sw, dmax, bmin, (double)nneg, (double)nviol, (double)nval, (double)nnf,
                                    (double)nbf};
-    double* out = part + (int64_t)jl * nch * kChunkStats + c;  // per source: kChunkStats arrays of nch
+    double* out = part + chunk_rec_index(jl, c, 0, nch);  // the record's kChunkStats values, 8 apart
 #pragma unroll
-    for (int q = 0; q < kChunkStats; ++q) out[(int64_t)q * nch] = r[q];
-    ch_lo[(int64_t)jl * nch + c] = __double2float_rd(lo);
-    ch_hi[(int64_t)jl * nch + c] = __double2float_ru(hi);
+    for (int q = 0; q < kChunkStats; ++q) out[8 * q] = r[q];
+    ch_lo[chunk_range_index(jl, c, nch)] = __double2float_rd(lo);
+    ch_hi[chunk_range_index(jl, c, nch)] = __double2float_ru(hi);
 }
 
 }  // namespace
