@@ -52,6 +52,7 @@ class Options(ctypes.Structure):
         ("split_depth", ctypes.c_int),
         ("band_cap", ctypes.c_int),
         ("graphs", ctypes.c_int),
+        ("tile_rows", ctypes.c_int),
     ]
 
 

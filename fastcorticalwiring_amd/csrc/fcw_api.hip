@@ -227,6 +227,8 @@ struct fcw_ctx {
     std::vector<int> fwd_tile_off, bwd_tile_off, level_off;
     // depth parts per level of the dense FP64 sweeps (1: not split)
     std::vector<int> fwd_split, bwd_split;
+    // row tile height per level of the forward and backward sweeps (64, 32 or 16)
+    std::vector<int> fwd_tm, bwd_tm;
     bool fwd_idx_lds = true, bwd_idx_lds = true;  // every node's sweep indices fit in LDS
     DBuf<double> panel_heat, panel_pois;
     DBuf<float> panel_pois32;  // FCW_FP32_IR: the Poisson panels rounded to FP32
@@ -375,8 +377,8 @@ struct fcw_ctx {
         return s_ * (s_ + 1.0) / 2.0 + s_ * b_;
     }
 
-    // Workgroups of a sweep launch over ntiles row tiles (the kernel's grid, one depth part).
-    int sweep_wgs(int ntiles) const { return (ntiles + 7) / 8 * 8 * (ldk / fcw::kTileN); }
+    // Workgroups of a sweep launch over ntiles row tiles of height tm (the kernel's grid, one depth part).
+    int sweep_wgs(int ntiles, int tm) const { return (ntiles + 7) / 8 * 8 * (ldk / (4096 / tm)); }
 
     void solve(Workspace& w, const double* panel, const BlockPlan* plan, double* Y, double* U, double* Dt = nullptr,
                int system = 0) {
@@ -406,8 +408,8 @@ struct fcw_ctx {
                 ntiles = fwd_tile_off[l + 1] - fwd_tile_off[l];
                 ent = level_entries[l];
             }
-            EventPair* e = ntiles > 0 ? ev_begin(w, kFwdGemm, system, l, sweep_wgs(ntiles), ent) : nullptr;
-            fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, node_start.p, node_s.p, node_b.p, node_poff.p,
+            EventPair* e = ntiles > 0 ? ev_begin(w, kFwdGemm, system, l, sweep_wgs(ntiles, fwd_tm[l]), ent) : nullptr;
+            fcw::launch_fwd_gemm(w.stream, tiles, ntiles, ldk, fwd_tm[l], node_start.p, node_s.p, node_b.p, node_poff.p,
                                  node_uoff.p, cown.p, ctail.p, panel, w.X.p, Y, U, fwd_idx_lds, l == 0,
                                  (int)sym.urows, plan ? 1 : fwd_split[l], w.part.p);
             ev_end(w, e);
@@ -417,9 +419,10 @@ struct fcw_ctx {
     void solve_bwd(Workspace& w, const double* panel, double* Y, double* Dt, int system) {
         const int L = sym.nlevels;
         for (int l = L - 1; l >= 0; --l) {
-            EventPair* e = ev_begin(w, kBwdGemm, system, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l]),
+            EventPair* e = ev_begin(w, kBwdGemm, system, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l], bwd_tm[l]),
                                     level_entries[l]);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
+                                 bwd_tm[l],
                                  node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, Y, w.Z.p,
                                  bwd_idx_lds, Dt, n, bwd_split[l], w.part.p);
             ev_end(w, e);
@@ -461,16 +464,18 @@ struct fcw_ctx {
     void solve32(Workspace& w, const PT* panel, double* acc64, int acc_mode, double* Dt = nullptr) {
         const int L = sym.nlevels;
         for (int l = 0; l < L; ++l) {
-            EventPair* e = ev_begin(w, kFwd32, 1, l, sweep_wgs(fwd_tile_off[l + 1] - fwd_tile_off[l]), level_entries[l]);
+            EventPair* e = ev_begin(w, kFwd32, 1, l, sweep_wgs(fwd_tile_off[l + 1] - fwd_tile_off[l], fwd_tm[l]),
+                                    level_entries[l]);
             fcw::launch_fwd_gemm(w.stream, fwd_tiles.p + fwd_tile_off[l], fwd_tile_off[l + 1] - fwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, cown.p, ctail.p, panel,
+                                 fwd_tm[l], node_start.p, node_s.p, node_b.p, node_poff.p, node_uoff.p, cown.p, ctail.p, panel,
                                  w.X32.p, w.Y32.p, w.U32.p, fwd_idx_lds, l == 0, (int)sym.urows);
             ev_end(w, e);
         }
         for (int l = L - 1; l >= 0; --l) {
-            EventPair* e = ev_begin(w, kBwd32, 1, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l]), level_entries[l]);
+            EventPair* e = ev_begin(w, kBwd32, 1, l, sweep_wgs(bwd_tile_off[l + 1] - bwd_tile_off[l], bwd_tm[l]),
+                                    level_entries[l]);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
-                                 node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
+                                 bwd_tm[l], node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
                                  w.Z32.p, bwd_idx_lds, acc64, acc_mode, Dt, n);
             ev_end(w, e);
         }
@@ -590,7 +595,7 @@ struct fcw_ctx {
                 for (int id : per_level[l]) {
                     nodes.push_back(id);
                     const int f = sym.node_s[id] + sym.node_b[id];
-                    for (int m0 = 0; m0 < f; m0 += fcw::kTileM) tiles.push_back({id, m0});
+                    for (int m0 = 0; m0 < f; m0 += fwd_tm[l]) tiles.push_back({id, m0});
                     ent += node_entries(id);
                 }
                 P.level_entries.push_back(ent);
@@ -776,6 +781,52 @@ struct fcw_ctx {
 };
 
 namespace {
+
+// Row tile height of one level's sweep launch: among 64, 32 and 16 (16: backward only) the height
+// the block width allows with the least modelled cost: the 16-deep depth steps every tile walks
+// (its depth range after the triangular skip), counted as the matrix-core work they execute (the
+// range rounded up to 4) plus half the per-thread staging loads of each step relative to the 8 of
+// a 64-row backward tile. Short tiles waste fewer rows on nodes with few own rows; tall tiles stage
+// fewer right-hand-side values per result. forced > 0 picks the smallest allowed height >= forced.
+int choose_tile_rows(const fcw::Symbolic& S, int l, bool forward, int ldk, int forced) {
+    static const int cand[3] = {16, 32, 64};
+    if (forced > 0) {
+        for (int tm : cand)
+            if (tm >= forced && fcw::sweep_tile_ok(forward, tm, ldk)) return tm;
+        return 64;
+    }
+    int pick = 64;
+    double best = 0.0;
+    bool have = false;
+    for (int tm : cand) {
+        if (!fcw::sweep_tile_ok(forward, tm, ldk)) continue;
+        const int tn = 4096 / tm;
+        const bool kids = forward && l > 0;  // forward non-leaf steps also stage two update rows
+        const double loads = (tm * fcw::kTileK + (kids ? 3 : 1) * tn * fcw::kTileK) / 256.0;
+        double cost = 0.0;
+        for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+            const int id = S.level_nodes[i];
+            const int sn = S.node_s[id], f = sn + S.node_b[id];
+            const int rows = forward ? f : sn, K = forward ? sn : f;
+            for (int m0 = 0; m0 < rows; m0 += tm) {
+                int kb = 0, ke = K;
+                if (!forward)
+                    kb = m0 / fcw::kTileK * fcw::kTileK;
+                else if (m0 + tm <= sn)
+                    ke = std::min(ke, m0 + tm);
+                if (ke <= kb) continue;
+                const double depth4 = (ke - kb + 3) / 4 * 4, steps = (ke - kb + fcw::kTileK - 1) / fcw::kTileK;
+                cost += (double)(ldk / tn) * (depth4 / fcw::kTileK + 0.5 * steps * loads / 8.0);
+            }
+        }
+        if (!have || cost < best) {
+            best = cost;
+            pick = tm;
+            have = true;
+        }
+    }
+    return pick;
+}
 
 void host_operators(int n, const double* V, int m, const int* F, std::vector<std::vector<std::pair<int, double>>>& rows,
                     std::vector<double>& mass) {
@@ -998,6 +1049,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->m = m;
         C->ldk = ((opt.block + fcw::kColPad - 1) / fcw::kColPad) * fcw::kColPad;
         if (opt.band_cap < 0) throw Fail("band_cap must be >= 0 (0 = default)");
+        if (opt.tile_rows != 0 && opt.tile_rows != 16 && opt.tile_rows != 32 && opt.tile_rows != 64)
+            throw Fail("tile_rows must be 0 (chosen per level), 16, 32 or 64");
         if (opt.band_cap > 0) C->band_cap = opt.band_cap;
         // the divergence addresses one RHS block with 32-bit byte offsets
         if ((uint64_t)n * (uint64_t)C->ldk * 8u >= (1ull << 32))
@@ -1172,36 +1225,30 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->node_boff.upload(S.node_boff.data(), S.node_boff.size(), tb);
         C->node_poff.upload(S.node_poff.data(), S.node_poff.size(), tb);
         C->node_uoff.upload(S.node_uoff.data(), S.node_uoff.size(), tb);
-        std::vector<fcw::LevelTile> ft, bt;
-        C->fwd_tile_off.assign(1, 0);
-        C->bwd_tile_off.assign(1, 0);
         C->level_off.assign(S.level_ptr.begin(), S.level_ptr.end());
-        for (int l = 0; l < S.nlevels; ++l) {
-            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
-                const int id = S.level_nodes[i];
-                const int f = S.node_s[id] + S.node_b[id];
-                for (int m0 = 0; m0 < f; m0 += fcw::kTileM) ft.push_back({id, m0});
-                for (int m0 = 0; m0 < S.node_s[id]; m0 += fcw::kTileM) bt.push_back({id, m0});
-            }
-            C->fwd_tile_off.push_back((int)ft.size());
-            C->bwd_tile_off.push_back((int)bt.size());
-        }
         C->level_entries.assign(S.nlevels, 0.0);
         for (int l = 0; l < S.nlevels; ++l)
             for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) C->level_entries[l] += C->node_entries(S.level_nodes[i]);
-        C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
-        C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
+        // Row tiles of height tm per level: forward over each node's s + b rows, backward over its s.
+        auto count_tiles = [&](int l, bool forward, int tm) {
+            int nt = 0;
+            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+                const int id = S.level_nodes[i];
+                nt += ((forward ? S.node_s[id] + S.node_b[id] : S.node_s[id]) + tm - 1) / tm;
+            }
+            return nt;
+        };
         C->fwd_split.assign(S.nlevels, 1);
         C->bwd_split.assign(S.nlevels, 1);
         const bool split = opt.split_depth > 0 || (opt.split_depth == 0 && opt.streams <= 2);
         if (split) {
             // Split the depth of a level's launch when it has few workgroups and a long depth: aim
-            // at about kSplitTarget workgroups, at least two depth steps per part, at most 8 parts.
-            // With several blocks in flight the narrow levels of one block already run beside the
-            // wide launches of others, and the split's extra workgroups cost throughput, so it is
-            // on by default only for one or two streams.
+            // at about kSplitTarget workgroups, at least two depth steps per part, at most 8 parts
+            // (64-row tiles). With several blocks in flight the narrow levels of one block already run
+            // beside the wide launches of others, and the split's extra workgroups cost throughput, so
+            // it is on by default only for one or two streams.
             constexpr int kSplitTarget = 512;
-            const int ncol = C->ldk / fcw::kTileN;
+            const int ncol = C->ldk / 64;
             auto choose = [&](int ntiles, int kmax) {
                 const int wgs = (ntiles + 7) / 8 * 8 * ncol;
                 if (ntiles <= 0 || wgs >= kSplitTarget) return 1;
@@ -1215,8 +1262,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
                     kf = std::max(kf, S.node_s[id]);
                     kb = std::max(kb, S.node_s[id] + S.node_b[id]);
                 }
-                const int nf = C->fwd_tile_off[l + 1] - C->fwd_tile_off[l];
-                const int nb = C->bwd_tile_off[l + 1] - C->bwd_tile_off[l];
+                const int nf = count_tiles(l, true, 64), nb = count_tiles(l, false, 64);
                 if (l > 0) C->fwd_split[l] = choose(nf, kf);  // the leaf level never splits
                 C->bwd_split[l] = choose(nb, kb);
                 if (C->fwd_split[l] > 1) part_elems = std::max(part_elems, (size_t)nf * ncol * C->fwd_split[l] * 4096);
@@ -1224,6 +1270,28 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
             }
             for (auto& w : C->ws) w->part.alloc(std::max<size_t>(1, part_elems), tb);
         }
+        // tile heights (a split level keeps 64-row tiles), then the tile lists
+        C->fwd_tm.assign(S.nlevels, 64);
+        C->bwd_tm.assign(S.nlevels, 64);
+        for (int l = 0; l < S.nlevels; ++l) {
+            if (C->fwd_split[l] == 1) C->fwd_tm[l] = choose_tile_rows(S, l, true, C->ldk, opt.tile_rows);
+            if (C->bwd_split[l] == 1) C->bwd_tm[l] = choose_tile_rows(S, l, false, C->ldk, opt.tile_rows);
+        }
+        std::vector<fcw::LevelTile> ft, bt;
+        C->fwd_tile_off.assign(1, 0);
+        C->bwd_tile_off.assign(1, 0);
+        for (int l = 0; l < S.nlevels; ++l) {
+            for (int i = S.level_ptr[l]; i < S.level_ptr[l + 1]; ++i) {
+                const int id = S.level_nodes[i];
+                const int f = S.node_s[id] + S.node_b[id];
+                for (int m0 = 0; m0 < f; m0 += C->fwd_tm[l]) ft.push_back({id, m0});
+                for (int m0 = 0; m0 < S.node_s[id]; m0 += C->bwd_tm[l]) bt.push_back({id, m0});
+            }
+            C->fwd_tile_off.push_back((int)ft.size());
+            C->bwd_tile_off.push_back((int)bt.size());
+        }
+        C->fwd_tiles.upload(ft.data(), std::max<size_t>(1, ft.size()), tb);
+        C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
         for (int l = 0; l < S.nlevels; ++l) C->info.split_levels += (C->fwd_split[l] > 1) + (C->bwd_split[l] > 1);
         const int max_s = S.nnodes ? *std::max_element(S.node_s.begin(), S.node_s.end()) : 0;
         C->fwd_idx_lds = max_s <= fcw::kSweepIdxCap;

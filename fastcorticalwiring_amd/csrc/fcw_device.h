@@ -56,9 +56,8 @@ struct LevelTile {
     int m0;
 };
 
-// Tiling of the level panel products (rows x columns x depth per workgroup).
-constexpr int kTileM = 64;
-constexpr int kTileN = 64;
+// Depth per step of the level panel products (the row tile height, 64, 32 or 16, is chosen per
+// level and direction; a workgroup's column slice is 4096 / height wide).
 constexpr int kTileK = 16;
 constexpr int kColPad = 64;  // RHS block widths are padded to this many columns
 // Per-node row indices staged in LDS by the sweeps: the forward sweep needs s <= kSweepIdxCap
@@ -211,14 +210,17 @@ void launch_set_delta(hipStream_t st, double* X, int ldk, const int* src_pos, in
 // Zero the own rows (in Y) and, with U set, the update rows of the listed supernodes.
 void launch_clear_rows(hipStream_t st, const int* nodes, int count, int ldk, const int* node_start,
                        const int* node_s, const int* node_b, const int64_t* node_uoff, double* Y, double* U);
-void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const double* panel, const double* X, double* Y,
                      double* U, bool idx_lds, bool leaves, int zrow, int nsplit = 1, double* part = nullptr);
-void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const double* panel, const double* Y, double* Z, bool idx_lds,
                      double* Dt, int ldd, int nsplit = 1, double* part = nullptr);
+// tm: row tile height of the launch's tiles (every tile's m0 a multiple of it): 64 (ldk a multiple
+// of 64), 32 (ldk a multiple of 128) or, backward only, 16 (ldk a multiple of 256); see sweep_tile_ok.
+bool sweep_tile_ok(bool forward, int tm, int ldk);
 // Forward sweeps: U has a row zrow past the update rows that stays zero (it stands in for a
 // missing child contribution, so every gather is an unconditional load). nsplit > 1 (FP64, not
 // the leaf level): the depth is split over nsplit workgroups per (tile, slice) whose partial
@@ -228,22 +230,22 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 // (unshifted; consumers subtract the source's own value).
 // FP32 forms of the sweeps (refined solve): FP32 panels and right-hand sides, FP32 matrix-core
 // accumulation; the backward sweep also stores (acc_mode 1) or adds (2) its solution into acc64.
-void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,
                      float* U, bool idx_lds, bool leaves, int zrow);
-void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start,
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_boff, const int* bidx,
                      const int64_t* node_poff, const float* panel, const float* Y, float* Z, bool idx_lds,
                      double* acc64, int acc_mode, double* Dt, int ldd);
 // Split-BF16 forms (refined solve, FCW_BF16X3_IR): panels packed as bf16 hi | lo << 16 (from the FP64
 // panels by launch_to_bf16x2), FP32 right-hand sides split while staged, three bf16 products per block.
 void launch_to_bf16x2(hipStream_t st, const double* src, uint32_t* dst, int64_t count);
-void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start, const int* node_s,
                      const int* node_b, const int64_t* node_poff, const int64_t* node_uoff, const int* contrib_own,
                      const int* contrib_tail, const uint32_t* panel, const float* X, float* Y, float* U, bool idx_lds,
                      bool leaves, int zrow);
-void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, const int* node_start, const int* node_s,
+void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start, const int* node_s,
                      const int* node_b, const int64_t* node_boff, const int* bidx, const int64_t* node_poff,
                      const uint32_t* panel, const float* Y, float* Z, bool idx_lds, double* acc64, int acc_mode,
                      double* Dt, int ldd);

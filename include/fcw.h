@@ -73,6 +73,10 @@ typedef struct {
                               block (the heat backward sweep; divergence + Poisson solve) into HIP graphs once
                               (built node by node, no stream capture) and replays them; 0: every launch issued
                               one by one */
+    int tile_rows;         /* row tile height of the panel sweeps: 0 (default) chooses per tree level and
+                              direction among 64, 32 and 16 by a model of the work each executes (32 needs a
+                              block width that is a multiple of 128; 16, backward only, a multiple of 256);
+                              16, 32 or 64: the smallest allowed height >= this on every level */
 } fcw_options;
 
 typedef struct {

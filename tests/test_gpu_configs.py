@@ -150,6 +150,38 @@ def test_refined_forms_match_fp64_on_every_source_of_the_largest_surface(form):
     assert max(worst.values()) < 1e-5, worst
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir", "bf16x3_ir"])
+@pytest.mark.parametrize("tile_rows", [64, 32, 16])
+def test_every_row_tile_height_matches_the_oracle(tile_rows, precision):
+    """The panel sweeps with one row tile height forced on every tree level (64 x 64, 32 x 128, and
+    backward 16 x 256 workgroup tiles; 256-source blocks allow all three) against the oracle, and
+    equal to the heights the library chooses per level: each result row accumulates the same
+    products in the same depth order whatever the tile height."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(1)
+    tol = TOL[precision]
+    with HeatContext(mesh.V, mesh.F, block=256, precision=precision, tile_rows=tile_rows) as ctx, \
+            HeatContext(mesh.V, mesh.F, block=256, precision=precision) as auto:
+        D = ctx.distances(fx["src"])
+        err, rel = check_distances(D, fx, tol)
+        assert err < tol["dist"] and rel < tol["dist"], (err, rel)
+        assert np.array_equal(D, auto.distances(fx["src"]))
+        set_metrics(ctx, fx, mesh)
+        w = ctx.wiring(fx["src"])
+    check_outputs(w["scalars"], w["radii"], w["perimeters"], w["extra_r"], w["extra_a"], w["sample_r"],
+                  w["sample_a"], fx, tol, f"tile_rows={tile_rows} {precision}")
+
+
+def test_tile_rows_option_is_validated():
+    from fastcorticalwiring_amd._lib import FcwError
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(1)
+    with pytest.raises(FcwError, match="tile_rows"):
+        HeatContext(mesh.V, mesh.F, tile_rows=48)
+
+
 def test_band_overflow_fallback_gives_the_same_metrics():
     """A band list that does not fit the per-source pool is evaluated from every face instead
     (the kernel's fallback); with a pool of 16 faces per scale most bands overflow, and the
