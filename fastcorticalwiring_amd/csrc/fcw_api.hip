@@ -1404,7 +1404,7 @@ int fcw_set_metrics(fcw_ctx* C, const double* targets, int S, const double* frac
         C->rtol = rtol;
         C->max_iter = max_iter;
         for (auto& w : C->ws) {
-            w->band.alloc((size_t)C->ldk * S * 5 * C->band_cap, tb);
+            w->band.alloc((size_t)C->ldk * fcw::band_pool_doubles((int64_t)S * C->band_cap), tb);
             w->cpart.alloc((size_t)C->ldk * std::max(1, C->nch) * fcw::kChunkStats, tb);
             w->ch_lo.alloc((size_t)C->ldk * std::max(1, C->nch), tb);
             w->ch_hi.alloc((size_t)C->ldk * std::max(1, C->nch), tb);

@@ -65,6 +65,9 @@ constexpr int kColPad = 64;  // RHS block widths are padded to this many columns
 // row per depth index).
 constexpr int kSweepIdxCap = 1024;
 
+// Doubles of band scratch per source for pool_cap band faces: a, b, c and the face index.
+__host__ __device__ inline int64_t band_pool_doubles(int64_t pool_cap) { return 3 * pool_cap + (pool_cap + 1) / 2; }
+
 // Metric kernel limits.
 constexpr int kMaxScales = 32;
 constexpr int kMaxExtras = 256;
@@ -155,7 +158,8 @@ struct MetricsDev {
     double rtol;
     int max_iter;
     int pool_cap;           // band faces per source held in the scratch (all bands together)
-    double* band;           // k x 5 x pool_cap (sorted a, b, c, area, |grad d| per band face)
+    double* band;           // per source band_pool_doubles(pool_cap): sorted a, b, c per band face (three
+                            // arrays of pool_cap doubles), then the faces' indices (pool_cap ints)
     // Far-field summaries from chunk_stats (null: the metric kernel makes its own vertex pass and
     // chunk bounds): per source and 256-face chunk, the distance range of the chunk's vertices
     // and kChunkStats partial sums over the vertices the chunk owns.

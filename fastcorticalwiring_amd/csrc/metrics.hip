@@ -206,7 +206,8 @@ __device__ inline double log_mix(double r0, double r1, double fr) { return exp(l
 
 // One band's sums at radius r over lanes of one wave: area inside r beyond the fully-inside
 // part and its derivative, and (with want_p) the coarea perimeter density. `pool` holds the
-// band's sorted (a, b, c), area and face index in five arrays of stride `pstride`; |grad d| is
+// band's sorted (a, b, c) in three arrays of stride `pstride` and, after them, the face indices
+// (ints); each face's area comes from the mesh (an array every source shares); |grad d| is
 // recomputed from the face for the faces that cross r. With `full`, the band is instead
 // recomputed from every face (the band did not fit in the scratch).
 struct BandRef {
@@ -228,18 +229,20 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
         const double* pa = pool + B.off;
         const double* pb = pa + pstride;
         const double* pc = pb + pstride;
-        const double* pA = pc + pstride;
-        const double* pf = pA + pstride;
+        const int* pf = reinterpret_cast<const int*>(pool + 3 * pstride) + B.off;
         for (int i0 = lane; i0 < B.count; i0 += U * kWave) {
             double a[U], b[U], c[U], A[U];
+            int f[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int i = min(i0 + u * kWave, B.count - 1);
                 a[u] = pa[i];
                 b[u] = pb[i];
                 c[u] = pc[i];
-                A[u] = pA[i];
+                f[u] = pf[i];
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u) A[u] = larea[f[u]];
             __builtin_amdgcn_sched_barrier(0);
             double dx[U];
 #pragma unroll
@@ -252,9 +255,7 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
                 dx[u] = ok ? dx[u] : 0.0;
             }
             if (want_p) {
-                int f[U], i1[U], i2[U], i3[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) f[u] = (int)pf[min(i0 + u * kWave, B.count - 1)];
+                int i1[U], i2[U], i3[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     i1[u] = faces[3 * f[u]];
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     double* er = out.extra_r + (int64_t)j * NE;
     double* ea = out.extra_a + (int64_t)j * NE;
     const int64_t pstride = md.pool_cap;
-    double* pool = md.band + (int64_t)j * 5 * pstride;
+    double* pool = md.band + (int64_t)j * band_pool_doubles(pstride);
 
     __shared__ Red red;
     __shared__ double Hmax[kMaxBins + 1];
@@ -805,8 +806,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                     bs[0] = sa_[u];
                     bs[pstride] = sb_[u];
                     bs[2 * pstride] = sc_[u];
-                    bs[3 * pstride] = fb.A[u];
-                    bs[4 * pstride] = (double)fb.f[u];
+                    reinterpret_cast<int*>(pool + 3 * pstride)[b_off[i] + slot] = fb.f[u];
                 }
             }
         }
