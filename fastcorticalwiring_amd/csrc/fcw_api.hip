@@ -217,7 +217,7 @@ struct fcw_ctx {
     DBuf<int> faces, vf_ptr, vf, rowptr, colind, cptr, contrib, iperm;
     // divergence row groups
     int div_ngroups = 0;
-    DBuf<int> div_grp, div_verts, div_rloc;
+    DBuf<int> div_grp, div_verts;
     DBuf<unsigned> div_loc;
     DBuf<double> div_geo;
     // tree
@@ -301,7 +301,6 @@ struct fcw_ctx {
         d.grp = reinterpret_cast<const int4*>(div_grp.p);
         d.verts = div_verts.p;
         d.loc = div_loc.p;
-        d.rloc = div_rloc.p;
         d.geo = div_geo.p;
         d.vf_ptr = vf_ptr.p;
         d.vf = vf.p;
@@ -880,11 +879,10 @@ double mean_edge_length(const fcw::Symbolic& S, const double* Vpos) {
 // kDivVerts distinct vertices and number at most kDivInc, and it has at most kDivGroupRows rows.
 // grp holds four ints per group (first row, first incidence, first entry in verts, distinct count;
 // -1 for a lone row beyond the limits) and a closing entry (n, total incidences, |verts|, 0);
-// loc[e] packs the local indices of incidence e's three face vertices, 6 bits each; rloc[p] is the
-// local index of row p's own vertex in its group (0 for a row without faces or a lone row).
+// loc[e] packs the local indices of incidence e's three face vertices, 6 bits each.
 void build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::vector<int>& vf,
                              const std::vector<int>& faces, std::vector<int>& grp, std::vector<int>& verts,
-                             std::vector<unsigned>& loc, std::vector<int>& rloc) {
+                             std::vector<unsigned>& loc) {
     std::vector<int> mark(n, -1), seen(n, -1), lidx(n, 0), cur;
     int r = 0, gid = 0, attempt = 0;  // a row that does not fit is tried again as the next group's first
     while (r < n) {
@@ -925,7 +923,6 @@ void build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::v
             const int* fv = faces.data() + 3 * (size_t)(vf[e] / 3);
             loc[e] = (unsigned)lidx[fv[0]] | ((unsigned)lidx[fv[1]] << 6) | ((unsigned)lidx[fv[2]] << 12);
         }
-        for (int q = r0; q < r; ++q) rloc[q] = vf_ptr[q + 1] > vf_ptr[q] ? lidx[q] : 0;
         grp.insert(grp.end(), {r0, e0, (int)verts.size(), (int)cur.size()});
         verts.insert(verts.end(), cur.begin(), cur.end());
         ++gid;
@@ -1142,9 +1139,7 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
             // divergence row groups over the operator triangulation's incidences
             std::vector<int> grp, verts;
             std::vector<unsigned> loc(std::max<size_t>(1, S.vf.size()), 0u);
-            std::vector<int> rloc(n, 0);
-            build_divergence_groups(n, S.vf_ptr, S.vf, S.faces, grp, verts, loc, rloc);
-            C->div_rloc.upload(rloc.data(), rloc.size(), tb);
+            build_divergence_groups(n, S.vf_ptr, S.vf, S.faces, grp, verts, loc);
             C->div_ngroups = (int)(grp.size() / 4) - 1;
             C->div_grp.upload(grp.data(), grp.size(), tb);
             C->div_verts.upload(verts.data(), std::max<size_t>(1, verts.size()), tb);

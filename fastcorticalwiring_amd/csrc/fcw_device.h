@@ -122,7 +122,6 @@ struct DivDev {
                            // (-1: one row beyond the limits, read from global memory); entry ngroups has .x = n
     const int* verts;      // distinct vertex positions of each group, ascending
     const unsigned* loc;   // per incidence: local indices of the face's vertices 0, 1, 2 (6 bits each)
-    const int* rloc;       // per row: local index of its own vertex in its group (0 for a row without faces)
     const double* geo;     // per incidence: A g1.ga, A g2.ga, |g1|^2, 2 g1.g2, |g2|^2
     const int* vf_ptr;     // n + 1, the operator triangulation's incidences per row
     const int* vf;         // face*3 + corner

@@ -4,6 +4,7 @@
 // n x ldk (one vertex per row, one source per column).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     __shared__ double sgeo[5][kDivInc];
     __shared__ unsigned sloc[kDivInc];
     __shared__ int rptr[kDivGroupRows + 1];
-    __shared__ int rloc[kDivGroupRows];  // local index of each row's own vertex
+    __shared__ int sexp[4][kDivSlice];   // per wave and column: largest exponent of its staged values
 
     const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
     if (item >= nitems) return;
@@ -244,7 +245,12 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     }
 
     if (tid <= nr) rptr[tid] = d.vf_ptr[r0 + tid] - e0;
-    if (tid < nr) rloc[tid] = d.rloc[r0 + tid];
+    // Each term is unchanged when the values are scaled by a power of two (exactly: the
+    // differences, the quadratic form and the reciprocal square root scale exactly), so each
+    // column's values are stored scaled by its largest magnitude in the group: the group spans a few
+    // mesh spacings, over which u changes by a few decades at most, so |grad u|^2 stays far above
+    // the range where the hardware estimate needs the rescaling of rsqrt_refined. A wave that meets
+    // such a value anyway (a vanishing gradient) sums the row again the general way.
     if (nv > 0) {
         // wave w loads vertices w, w + 4, ...: all its loads first (in-range repeats of the last
         // vertex pad the count), then the LDS stores
@@ -253,9 +259,20 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
         const int* gv = d.verts + v0;
 #pragma unroll
         for (int q = 0; q < Q; ++q) x[q] = u[(int64_t)gv[min(w + 4 * q, nv - 1)] * ldk + col];
+        int emax = INT_MIN;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            int ex = 0;
+            (void)frexp(x[q], &ex);
+            emax = (w + 4 * q < nv && x[q] != 0.0) ? max(emax, ex) : emax;
+        }
+        sexp[w][lane] = emax;
+        __syncthreads();
+        emax = max(max(sexp[0][lane], sexp[1][lane]), max(sexp[2][lane], sexp[3][lane]));
+        const double sc = ldexp(1.0, emax == INT_MIN ? 0 : -emax);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (w + 4 * q < nv) vals[w + 4 * q][lane] = x[q];
+            if (w + 4 * q < nv) vals[w + 4 * q][lane] = x[q] * sc;
     }
     {
         const int ni = d.vf_ptr[r0 + nr] - e0;
@@ -269,14 +286,6 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     __syncthreads();
     for (int r = w; r < nr; r += 4) {
         const int rs = rptr[r], re = rptr[r + 1];
-        // Each term is unchanged when the values are scaled by a power of two (exactly: the
-        // differences, the quadratic form and the reciprocal square root scale exactly), so the
-        // row's values are scaled by its own vertex's magnitude: then |grad u|^2 stays far above
-        // the range where the hardware estimate needs the rescaling of rsqrt_refined. A wave that
-        // meets such a value anyway (a vanishing gradient) sums the row again the general way.
-        int ex = 0;
-        (void)frexp(vals[rloc[r]][lane], &ex);
-        const double sc = ldexp(1.0, -ex);
         double acc = 0.0;
         bool tiny = false;
         // kDivC incidences per step, all computed (the padding repeats the last one), then added
@@ -288,8 +297,8 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
                 const int ec = min(e + q, re - 1);
                 const unsigned L = sloc[ec];
                 const double u0 = vals[L & 63u][lane];
-                const double du1 = (vals[(L >> 6) & 63u][lane] - u0) * sc;
-                const double du2 = (vals[(L >> 12) & 63u][lane] - u0) * sc;
+                const double du1 = vals[(L >> 6) & 63u][lane] - u0;
+                const double du2 = vals[(L >> 12) & 63u][lane] - u0;
                 const double num = fma(du2, sgeo[1][ec], du1 * sgeo[0][ec]);
                 const double qq = fma(du1, fma(du2, sgeo[3][ec], du1 * sgeo[2][ec]), du2 * (du2 * sgeo[4][ec]));
                 tiny |= qq > 0.0 && qq < 0x1p-767;
