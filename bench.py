@@ -182,11 +182,13 @@ def finish_cpu_leg(leg, n_sources):
 # --------------------------------------------------------------------------------------------
 # GPU measurement
 # --------------------------------------------------------------------------------------------
-def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, check_src, check_d_src, leaf_size=64):
+def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, check_src, check_d_src, leaf_size=64,
+            tile_rows=0):
     from fastcorticalwiring_amd.engine import HeatContext
 
     t0 = time.perf_counter()
-    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision, streams=streams, leaf_size=leaf_size)
+    ctx = HeatContext(prep["V"], prep["F"], block=block, precision=precision, streams=streams, leaf_size=leaf_size,
+                      tile_rows=tile_rows)
     ctx.set_metrics(prep["targets"], prep["fracs"], prep["boundary"], prep["vertex_areas"],
                     health_h=prep["health_h"], bin_width=prep["width"])
     setup = time.perf_counter() - t0
@@ -233,7 +235,7 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, chec
     return out
 
 
-def hemisphere_latency(prep, precision, block, streams, leaf_size=64):
+def hemisphere_latency(prep, precision, block, streams, leaf_size=64, tile_rows=0):
     """One whole hemisphere from the surface in host memory to every source's outputs in host
     memory: device mesh preparation, analysis and both factorisations, all sources, copy back."""
     from fastcorticalwiring_amd.serving import HemisphereRequest, ServingEngine
@@ -241,7 +243,7 @@ def hemisphere_latency(prep, precision, block, streams, leaf_size=64):
     s = prep["surface"]
     req = HemisphereRequest(0, 0.0, s.vertices, s.faces, s.cortex_mask)
     eng = ServingEngine(max_active=1, block=block, precision=precision, keep_outputs=False, streams=streams,
-                        chunk_blocks=1 << 20, leaf_size=leaf_size)
+                        chunk_blocks=1 << 20, leaf_size=leaf_size, tile_rows=tile_rows)
     r = eng.run([req])[0]
     if r.error:
         raise RuntimeError(r.error)
@@ -393,16 +395,16 @@ def main():
         _lib.check(_lib.lib().fcw_calibration_copy(int(args.calibrate_bytes), 0))
     forms = ["fp64", "fp32_ir", "bf16x3_ir"] if args.precision == "auto" else [args.precision]
     tuned = None
-    block, leaf = args.block, args.leaf_size
+    block, leaf, tile = args.block, args.leaf_size, 0
     if args.autotune == "on":
         from fastcorticalwiring_amd import autotune
 
-        log("bench: autotuning block width and leaf size")
+        log("bench: autotuning block width, leaf size and sweep tile height")
         best_cfg, trials = autotune.autotune(prep["V"], prep["F"], blocks=(128, 256, 512), leaf_sizes=(32, 64, 96, 128),
                                              precision=forms[0], streams=args.streams, order=prep["order"],
-                                             n_sources=min(prep["order"].size, 16384), log=log)
-        block, leaf = best_cfg["block"], best_cfg["leaf_size"]
-        tuned = dict(block=block, leaf_size=leaf, precision=forms[0], trials=trials)
+                                             n_sources=min(prep["order"].size, 16384), log=log, tile_rows=(0, 64, 32))
+        block, leaf, tile = best_cfg["block"], best_cfg["leaf_size"], best_cfg["tile_rows"]
+        tuned = dict(block=block, leaf_size=leaf, tile_rows=tile, precision=forms[0], trials=trials)
     check_src = cpu_sample(n, args.cpu_sources)  # the CPU leg's sample, checked on the GPU as well
     nw = leg["nworkers"] if leg else 1
     check_d_src = np.concatenate([p[:CPU_KEEP_FIELDS] for p in np.array_split(check_src, nw)])
@@ -410,7 +412,7 @@ def main():
     for form in forms:
         log(f"bench: timing {form}")
         results[form] = run_gpu(prep, form, block, args.steps, args.warmup, args.profile_blocks,
-                                args.streams, check_src, check_d_src, leaf)
+                                args.streams, check_src, check_d_src, leaf, tile)
         r = results[form]
         log(f"bench: {form}: {r['sources_per_s']:.1f} sources/s ({r['ms_per_step']:.2f} ms/block), "
             f"setup {r['setup_s']:.2f}s")
@@ -421,7 +423,7 @@ def main():
     latency = {}
     if args.latency == "on":
         for form in forms:
-            latency[form] = hemisphere_latency(prep, form, block, args.streams, leaf)
+            latency[form] = hemisphere_latency(prep, form, block, args.streams, leaf, tile)
             log(f"bench: {form}: one hemisphere end to end {latency[form]['latency_s']:.2f}s")
     best = max(results.values(), key=lambda r: r["sources_per_s"])
     serving = None
@@ -494,6 +496,7 @@ def main():
             "faces": int(prep["F"].shape[0]),
             "block": int(best["info"]["block"]),
             "leaf_size": int(leaf),
+            "tile_rows": int(tile) if tile else "chosen per tree level",
             "streams": int(args.streams),
             "precision": best["precision"],
             "scales": 12,

@@ -76,7 +76,8 @@ class WiringAnalysis:
                                leaf_size=int(kw.pop("leaf_size", 64)), precision=str(kw.pop("precision", "fp64")),
                                ir_steps=kw.pop("ir_steps", None), device=int(kw.pop("device", 0)),
                                host_threads=int(kw.pop("host_threads", 0)), streams=int(kw.pop("streams", 8)),
-                               robust=self.use_robust, split_depth=int(kw.pop("split_depth", 0)))
+                               robust=self.use_robust, split_depth=int(kw.pop("split_depth", 0)),
+                               tile_rows=int(kw.pop("tile_rows", 0)))
         if kw:
             raise ValueError(f"Unsupported engine kwargs: {', '.join(sorted(kw))}")
         self.field_flag = np.zeros(self.n_vertices_full, dtype=np.uint8)

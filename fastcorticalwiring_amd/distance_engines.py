@@ -62,14 +62,15 @@ def _backend_mixin(base, potpourri=False):
           streams (1: this adapter solves one block per miss, so more workspaces would sit idle),
           use_robust (False: operators on the surface's own triangulation, as ConsistentBatchHeatEngine;
           True: on its intrinsic Delaunay triangulation, as potpourri3d's robust mode),
-          split_depth (0: narrow tree levels split their depth when streams <= 2, 1 always, -1 never).
+          split_depth (0: narrow tree levels split their depth when streams <= 2, 1 always, -1 never),
+          tile_rows (0: sweep row tile heights chosen per tree level; 16, 32 or 64 on every level).
         Sources are solved in RHS blocks: a miss solves the next ``block`` sources of the planned
         order (default: the reference's BFS order, core_analysis.py:1384-1417), so the caller's
         one-source-at-a-time loop reads from the cache.
         """
 
         _ALLOWED = {"t_coef", "diffusion_length_mm", "poisson_shift", "block", "precision", "leaf_size", "device",
-                    "ir_steps", "host_threads", "streams", "use_robust", "split_depth"} | (
+                    "ir_steps", "host_threads", "streams", "use_robust", "split_depth", "tile_rows"} | (
                         {"allow_eigen_fallback"} if potpourri else set())
         _NAME = POTPOURRI_NAME if potpourri else ENGINE_NAME
 
@@ -135,7 +136,7 @@ def _backend_mixin(base, potpourri=False):
                                     precision=precision, ir_steps=kw.pop("ir_steps", None),
                                     device=int(kw.pop("device", 0)), host_threads=int(kw.pop("host_threads", 0)),
                                     streams=int(kw.pop("streams", 1)), robust=robust,
-                                    split_depth=int(kw.pop("split_depth", 0)))
+                                    split_depth=int(kw.pop("split_depth", 0)), tile_rows=int(kw.pop("tile_rows", 0)))
             self.t = self._ctx.t
             self._plan = None
             self._plan_pos = None

@@ -96,7 +96,8 @@ class Admission:
 
 
 class _Prepared:
-    def __init__(self, req: HemisphereRequest, block, streams, host_threads, precision, leaf_size=64, tuner=None):
+    def __init__(self, req: HemisphereRequest, block, streams, host_threads, precision, leaf_size=64, tuner=None,
+                 tile_rows=0):
         from .engine import HeatContext
 
         from . import _lib
@@ -106,11 +107,11 @@ class _Prepared:
         self.V, self.F, self.s2o, self.o2s = d["vertices"], d["faces"], d["sub_to_orig"], d["orig_to_sub"]
         fl, va = d["face_L"], d["vertex_areas"]
         self.order = d["order"]
-        if tuner is not None:  # the tuned block width and leaf size of this mesh's size class
+        if tuner is not None:  # the tuned block width, leaf size (and tile height) of this mesh's size class
             cfg = tuner.choose(self.V, self.F, self.order)
-            block, leaf_size = cfg["block"], cfg["leaf_size"]
+            block, leaf_size, tile_rows = cfg["block"], cfg["leaf_size"], cfg.get("tile_rows", tile_rows)
         self.ctx = HeatContext(self.V, self.F, block=block, precision=precision, host_threads=host_threads,
-                               streams=streams, leaf_size=leaf_size)
+                               streams=streams, leaf_size=leaf_size, tile_rows=tile_rows)
         self.ctx_streams = streams
         targets = np.array(sorted(req.scales)) * float(va.sum())
         q = int(req.n_samples_between_scales)
@@ -129,12 +130,13 @@ class ServingEngine:
     streams          : RHS blocks in flight per hemisphere context (the active contexts share the
                        device's hardware queues)
     leaf_size        : nested-dissection leaf size of every context
+    tile_rows        : sweep row tile height of every context (0: chosen per tree level)
     tuner            : an ``autotune.Tuner``: block width and leaf size per mesh size class instead
     """
 
     def __init__(self, hbm_budget_bytes=96 << 30, max_active=3, block=256, chunk_blocks=8, prep_threads=2,
                  host_threads=8, precision="fp64", keep_outputs=True, prepare=None, streams=4, on_done=None,
-                 leaf_size=64, tuner=None):
+                 leaf_size=64, tuner=None, tile_rows=0):
         self.hbm_budget = int(hbm_budget_bytes)
         self.max_active = int(max_active)
         self.block = int(block)
@@ -145,9 +147,11 @@ class ServingEngine:
         self.keep_outputs = keep_outputs
         self.streams = int(streams)
         self.leaf_size = int(leaf_size)
+        self.tile_rows = int(tile_rows)
         self.tuner = tuner
         self._prepare = prepare or (lambda req: _Prepared(req, self.block, self.streams, self.host_threads,
-                                                           self.precision, self.leaf_size, self.tuner))
+                                                           self.precision, self.leaf_size, self.tuner,
+                                                           self.tile_rows))
         self._admission = Admission(self.max_active, self.hbm_budget)
         self.trace = []
         self.on_done = on_done  # called with each RequestResult as it completes
