@@ -220,6 +220,11 @@ struct fcw_ctx {
     DBuf<int> div_grp, div_verts;
     DBuf<unsigned> div_loc;
     DBuf<double> div_geo;
+    // refined forms: the residual's matrix values (L + shift M) and CSR entries' local indices in
+    // the divergence groups, when every row's columns lie in its group (res_grouped)
+    DBuf<unsigned char> res_cloc;
+    DBuf<double> res_aval;
+    bool res_grouped = false;
     // tree
     DBuf<int> node_start, node_s, node_b, bidx, cmap, child_ptr, child_list, level_nodes, cown, ctail;
     DBuf<int64_t> node_boff, node_poff, node_uoff;
@@ -498,8 +503,12 @@ struct fcw_ctx {
         solve32(w, panel, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
             e = ev_begin(w, kResidual, 1);
-            fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p,
-                                   w.Z.p, w.X32.p);
+            if (res_grouped)
+                fcw::launch_residual_grouped(w.stream, div_dev(), ldk, rowptr.p, colind.p, res_cloc.p, res_aval.p,
+                                             w.X.p, w.Z.p, w.X32.p);
+            else
+                fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift,
+                                       w.X.p, w.Z.p, w.X32.p);
             ev_end(w, e);
             solve32(w, panel, w.Z.p, 2, it + 1 == opt.ir_steps ? w.Dt.p : nullptr);
         }
@@ -879,10 +888,14 @@ double mean_edge_length(const fcw::Symbolic& S, const double* Vpos) {
 // kDivVerts distinct vertices and number at most kDivInc, and it has at most kDivGroupRows rows.
 // grp holds four ints per group (first row, first incidence, first entry in verts, distinct count;
 // -1 for a lone row beyond the limits) and a closing entry (n, total incidences, |verts|, 0);
-// loc[e] packs the local indices of incidence e's three face vertices, 6 bits each.
-void build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::vector<int>& vf,
+// loc[e] packs the local indices of incidence e's three face vertices, 6 bits each. With a CSR
+// pattern (rowptr, colind), cloc[e] is the local index of CSR entry e's column in its row's group;
+// returns false if some column is not among its group's vertices (then cloc is not usable).
+bool build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::vector<int>& vf,
                              const std::vector<int>& faces, std::vector<int>& grp, std::vector<int>& verts,
-                             std::vector<unsigned>& loc) {
+                             std::vector<unsigned>& loc, const std::vector<int>* rowptr = nullptr,
+                             const std::vector<int>* colind = nullptr, std::vector<unsigned char>* cloc = nullptr) {
+    bool cloc_ok = true;
     std::vector<int> mark(n, -1), seen(n, -1), lidx(n, 0), cur;
     int r = 0, gid = 0, attempt = 0;  // a row that does not fit is tried again as the next group's first
     while (r < n) {
@@ -923,11 +936,19 @@ void build_divergence_groups(int n, const std::vector<int>& vf_ptr, const std::v
             const int* fv = faces.data() + 3 * (size_t)(vf[e] / 3);
             loc[e] = (unsigned)lidx[fv[0]] | ((unsigned)lidx[fv[1]] << 6) | ((unsigned)lidx[fv[2]] << 12);
         }
+        if (cloc)
+            for (int q = r0; q < r; ++q)
+                for (int e = (*rowptr)[q]; e < (*rowptr)[q + 1]; ++e) {
+                    const int c = (*colind)[e];
+                    if (mark[c] != gid) cloc_ok = false;
+                    (*cloc)[e] = (unsigned char)(mark[c] == gid ? lidx[c] : 0);
+                }
         grp.insert(grp.end(), {r0, e0, (int)verts.size(), (int)cur.size()});
         verts.insert(verts.end(), cur.begin(), cur.end());
         ++gid;
     }
     grp.insert(grp.end(), {n, vf_ptr[n], (int)verts.size(), 0});
+    return cloc_ok;
 }
 
 // Mean length of the surface's distinct edges (distance_engines.py:17-21).
@@ -1139,7 +1160,18 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
             // divergence row groups over the operator triangulation's incidences
             std::vector<int> grp, verts;
             std::vector<unsigned> loc(std::max<size_t>(1, S.vf.size()), 0u);
-            build_divergence_groups(n, S.vf_ptr, S.vf, S.faces, grp, verts, loc);
+            // the refined forms' residual reads x through the same groups
+            const bool refined = opt.precision != FCW_FP64;
+            std::vector<unsigned char> cloc(refined ? std::max<size_t>(1, S.colind.size()) : 0);
+            const bool cloc_ok = build_divergence_groups(n, S.vf_ptr, S.vf, S.faces, grp, verts, loc, &S.rowptr,
+                                                         &S.colind, refined ? &cloc : nullptr);
+            if (refined && cloc_ok) {
+                C->res_cloc.upload(cloc.data(), cloc.size(), tb);
+                C->res_aval.alloc(std::max<size_t>(1, S.colind.size()), tb);
+                fcw::launch_shifted_values(st, n, C->rowptr.p, C->colind.p, C->lval.p, C->mass.p, opt.poisson_shift,
+                                           C->res_aval.p);
+                C->res_grouped = true;
+            }
             C->div_ngroups = (int)(grp.size() / 4) - 1;
             C->div_grp.upload(grp.data(), grp.size(), tb);
             C->div_verts.upload(verts.data(), std::max<size_t>(1, verts.size()), tb);

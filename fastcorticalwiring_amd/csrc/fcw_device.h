@@ -255,6 +255,13 @@ void launch_bwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk
 // r = b - (L + shift M) x in FP64 (x null: r = b), rounded to FP32 (the refined solve's RHS).
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r);
+// The same residual with x staged per divergence row group in LDS (x non-null): aval = L + shift M in
+// CSR order (launch_shifted_values), cloc = each CSR entry's local vertex index in its row's group.
+void launch_residual_grouped(hipStream_t st, const DivDev& d, int ldk, const int* rowptr, const int* colind,
+                             const unsigned char* cloc, const double* aval, const double* b, const double* x,
+                             float* r);
+void launch_shifted_values(hipStream_t st, int n, const int* rowptr, const int* colind, const double* lval,
+                           const double* mass, double shift, double* aval);
 void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count);
 // Per-incidence coefficients of the divergence, computed once per mesh (see DivDev::geo).
 void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const double* area, const double* gram,

@@ -357,6 +357,81 @@ __global__ __launch_bounds__(256) void residual32_kernel(int n, int ldk, const i
     r[(int64_t)row * ldk + col] = (float)v;
 }
 
+// The same residual over the divergence's row groups: a workgroup stages the values of x at the
+// group's distinct vertices for one 64-column slice in LDS (every row's matrix neighbours are
+// vertices of its faces, so they are among them), then each wave takes rows of the group, one lane
+// per column, reading x only from LDS. aval = the CSR values of L + shift M (shift on the
+// diagonal); cloc = each CSR entry's local vertex index in its row's group. Sums in CSR order, as
+// residual32_kernel.
+__global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int ncs, int run, DivDev d,
+                                                               const int* __restrict__ rowptr,
+                                                               const int* __restrict__ colind,
+                                                               const unsigned char* __restrict__ cloc,
+                                                               const double* __restrict__ aval,
+                                                               const double* __restrict__ b,
+                                                               const double* __restrict__ x, float* __restrict__ r,
+                                                               int ldk) {
+    __shared__ double vals[kDivVerts][kDivSlice];
+    const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
+    if (item >= nitems) return;
+    const int g = item / ncs, cs = item - g * ncs;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int4 G = d.grp[g];
+    const int r0 = G.x, v0 = G.z, nv = G.w;
+    const int nr = d.grp[g + 1].x - r0;
+    const int col = cs * kDivSlice + lane;
+    if (nv < 0) {
+        // a row beyond the group limits, alone: x straight from global memory
+        if (w != 0) return;
+        double ax = 0.0;
+        for (int e = rowptr[r0]; e < rowptr[r0 + 1]; ++e) ax += aval[e] * x[(int64_t)colind[e] * ldk + col];
+        r[(int64_t)r0 * ldk + col] = (float)(b[(int64_t)r0 * ldk + col] - ax);
+        return;
+    }
+    if (nv > 0) {
+        constexpr int Q = kDivVerts / 4;
+        double xv[Q];
+        const int* gv = d.verts + v0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) xv[q] = x[(int64_t)gv[min(w + 4 * q, nv - 1)] * ldk + col];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (w + 4 * q < nv) vals[w + 4 * q][lane] = xv[q];
+    }
+    __syncthreads();
+    for (int rr = w; rr < nr; rr += 4) {
+        const int p = r0 + rr;
+        const int e0 = rowptr[p], e1 = rowptr[p + 1];
+        const double bv = b[(int64_t)p * ldk + col];
+        double ax = 0.0;
+        constexpr int kC = 8;
+        for (int e = e0; e < e1; e += kC) {
+            double a[kC], xq[kC];
+            unsigned c[kC];
+#pragma unroll
+            for (int q = 0; q < kC; ++q) {
+                const int ec = min(e + q, e1 - 1);
+                a[q] = aval[ec];
+                c[q] = cloc[ec];
+            }
+#pragma unroll
+            for (int q = 0; q < kC; ++q) xq[q] = vals[c[q]][lane];
+#pragma unroll
+            for (int q = 0; q < kC; ++q) ax += (e + q < e1) ? a[q] * xq[q] : 0.0;
+        }
+        r[(int64_t)p * ldk + col] = (float)(bv - ax);
+    }
+}
+
+// aval[e] = lval[e] plus shift * mass[p] on row p's diagonal entry (the residual's matrix values).
+__global__ void shifted_values_kernel(int n, const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                      const double* __restrict__ lval, const double* __restrict__ mass, double shift,
+                                      double* __restrict__ aval) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    for (int e = rowptr[p]; e < rowptr[p + 1]; ++e) aval[e] = lval[e] + (colind[e] == p ? shift * mass[p] : 0.0);
+}
+
 __global__ void to_float_kernel(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = (float)src[i];
@@ -432,6 +507,24 @@ void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
                        const double* mass, double shift, const double* b, const double* x, float* r) {
     fcw::launch(residual32_kernel, dim3(dim3((n + 3) / 4, ldk / 64)), dim3(256), 0, st, n, ldk, rowptr, colind, lval, mass, shift, b, x, r);
+}
+
+void launch_residual_grouped(hipStream_t st, const DivDev& d, int ldk, const int* rowptr, const int* colind,
+                             const unsigned char* cloc, const double* aval, const double* b, const double* x,
+                             float* r) {
+    if (d.ngroups <= 0) return;
+    const int ncs = ldk / kDivSlice;
+    const int nitems = d.ngroups * ncs;
+    const int run = (nitems + kXcds - 1) / kXcds;  // items per XCD
+    fcw::launch(residual_grouped_kernel, dim3(dim3(run * kXcds)), dim3(256), 0, st, nitems, ncs, run, d, rowptr,
+                colind, cloc, aval, b, x, r, ldk);
+}
+
+void launch_shifted_values(hipStream_t st, int n, const int* rowptr, const int* colind, const double* lval,
+                           const double* mass, double shift, double* aval) {
+    if (n <= 0) return;
+    fcw::launch(shifted_values_kernel, dim3(grid1(n, 256)), dim3(256), 0, st, n, rowptr, colind, lval, mass, shift,
+                aval);
 }
 
 void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t count) {
