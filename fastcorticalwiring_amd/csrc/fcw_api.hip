@@ -480,7 +480,7 @@ struct fcw_ctx {
                                     level_entries[l]);
             fcw::launch_bwd_gemm(w.stream, bwd_tiles.p + bwd_tile_off[l], bwd_tile_off[l + 1] - bwd_tile_off[l], ldk,
                                  bwd_tm[l], node_start.p, node_s.p, node_b.p, node_boff.p, bidx.p, node_poff.p, panel, w.Y32.p,
-                                 w.Z32.p, bwd_idx_lds, acc64, acc_mode, Dt, n);
+                                 w.Z32.p, bwd_idx_lds, acc64, acc_mode | (l == 0 ? 4 : 0), Dt, n);
             ev_end(w, e);
         }
     }

@@ -232,7 +232,8 @@ bool sweep_tile_ok(bool forward, int tm, int ldk);
 // Backward sweeps with Dt set also write the FP64 solution source-major, Dt[col * ldd + position]
 // (unshifted; consumers subtract the source's own value).
 // FP32 forms of the sweeps (refined solve): FP32 panels and right-hand sides, FP32 matrix-core
-// accumulation; the backward sweep also stores (acc_mode 1) or adds (2) its solution into acc64.
+// accumulation; the backward sweep also stores (acc_mode 1) or adds (2) its solution into acc64, and
+// with 4 added to acc_mode does not store the FP32 solution itself (rows no other node reads).
 void launch_fwd_gemm(hipStream_t st, const LevelTile* tiles, int ntiles, int ldk, int tm, const int* node_start,
                      const int* node_s, const int* node_b, const int64_t* node_poff, const int64_t* node_uoff,
                      const int* contrib_own, const int* contrib_tail, const float* panel, const float* X, float* Y,

@@ -91,8 +91,10 @@ enum class Sweep { Forward, Backward };
 // T: value type of panels, right-hand sides and solution (double, or float with FP32
 // accumulation for the refined solve). KIDS = false: every node of the launch is a leaf (no
 // child update rows to gather). Backward, acc_mode 1 / 2 also stores / adds each solution value
-// into the FP64 array acc64 (the iterate of the refined solve), and with Dt set the (FP64)
-// solution value is also written source-major, Dt[col * ldd + position], for the metric kernel.
+// into the FP64 array acc64 (the iterate of the refined solve); with 4 added the solution itself is
+// not stored (the refined solve's leaf level: no node reads leaf rows of Z, and the iterate is
+// acc64); with Dt set the (FP64) solution value is also written source-major,
+// Dt[col * ldd + position], for the metric kernel.
 //
 // Grid: one workgroup per (row tile, TN-column slice), laid out in 1-D so that each XCD
 // (workgroups go to the 8 XCDs round robin) walks its own contiguous run of tiles, the column
@@ -429,10 +431,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
         if (row >= M) return 0.0;
         const T v = acc[i][jj][r];
         const int64_t o = (int64_t)(start + row) * ldk + col;
-        Out[o] = v;
+        if (!(acc_mode & 4)) Out[o] = v;
         double x = (double)v;
-        if (acc_mode == 1) acc64[o] = x;
-        if (acc_mode == 2) {
+        if ((acc_mode & 3) == 1) acc64[o] = x;
+        if ((acc_mode & 3) == 2) {
             x += acc64[o];
             acc64[o] = x;
         }
