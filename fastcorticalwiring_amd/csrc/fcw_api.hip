@@ -494,12 +494,10 @@ struct fcw_ctx {
             poisson_refined_with(w, panel_pois32.p);
     }
 
+    // The first FP32 right-hand side, w.X32 = w.X rounded to FP32, is written by the divergence.
     template <typename PT>
     void poisson_refined_with(Workspace& w, const PT* panel) {
-        EventPair* e = ev_begin(w, kResidual, 1);
-        fcw::launch_residual32(w.stream, n, ldk, rowptr.p, colind.p, lval.p, mass.p, opt.poisson_shift, w.X.p, nullptr,
-                               w.X32.p);
-        ev_end(w, e);
+        EventPair* e = nullptr;
         solve32(w, panel, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
             e = ev_begin(w, kResidual, 1);
@@ -542,7 +540,8 @@ struct fcw_ctx {
         if (profiling) HIPCHK(hipEventRecord(ev[1], w.stream));
         auto divergence = [&] {
             EventPair* ed = ev_begin(w, kDivergence);
-            fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p);
+            // the refined forms' first FP32 right-hand side is the divergence rounded to FP32
+            fcw::launch_divergence(w.stream, div_dev(), ldk, w.Z.p, w.X.p, opt.precision != FCW_FP64 ? w.X32.p : nullptr);
             ev_end(w, ed);
         };
         auto poisson = [&] {

@@ -267,7 +267,8 @@ void launch_to_float(hipStream_t st, const double* src, float* dst, int64_t coun
 // Per-incidence coefficients of the divergence, computed once per mesh (see DivDev::geo).
 void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const double* area, const double* gram,
                                double* geo);
-void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out);
+// out32 (optional): the same values rounded to FP32 (the refined solve's first right-hand side).
+void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out, float* out32 = nullptr);
 // out[j][v] = Dt[j][iperm[v]] - phi[src_pos[j]][j]: fields in caller vertex order, 0 at the source.
 void launch_gather_rows(hipStream_t st, int n, int ldk, int k, const double* Dt, const double* phi, const int* src_pos,
                         const int* iperm, double* out);

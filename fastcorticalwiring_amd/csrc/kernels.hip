@@ -211,7 +211,7 @@ __global__ void incidence_geometry_kernel(int ninc, const int* __restrict__ vf, 
 }
 
 __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, int run, DivDev d, const double* __restrict__ u,
-                                                         double* __restrict__ out, int ldk) {
+                                                         double* __restrict__ out, int ldk, float* __restrict__ out32) {
     __shared__ double vals[kDivVerts][kDivSlice];  // the group's distinct vertex rows, this slice
     __shared__ double sgeo[5][kDivInc];
     __shared__ unsigned sloc[kDivInc];
@@ -241,6 +241,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             acc += divergence_term(u0, u1, u2, d.geo + 5 * (int64_t)e);
         }
         out[(int64_t)r0 * ldk + col] = acc;
+        if (out32) out32[(int64_t)r0 * ldk + col] = (float)acc;
         return;
     }
 
@@ -319,6 +320,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             }
         }
         out[(int64_t)(r0 + r) * ldk + col] = acc;
+        if (out32) out32[(int64_t)(r0 + r) * ldk + col] = (float)acc;
     }
 }
 
@@ -496,12 +498,12 @@ void launch_incidence_geometry(hipStream_t st, int ninc, const int* vf, const do
     fcw::launch(incidence_geometry_kernel, dim3(grid1(ninc, 256)), dim3(256), 0, st, ninc, vf, area, gram, geo);
 }
 
-void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out) {
+void launch_divergence(hipStream_t st, const DivDev& d, int ldk, const double* u, double* out, float* out32) {
     if (d.ngroups <= 0) return;
     const int ncs = ldk / kDivSlice;
     const int nitems = d.ngroups * ncs;
     const int run = (nitems + kXcds - 1) / kXcds;  // items per XCD
-    fcw::launch(divergence_kernel, dim3(dim3(run * kXcds)), dim3(256), 0, st, nitems, ncs, run, d, u, out, ldk);
+    fcw::launch(divergence_kernel, dim3(dim3(run * kXcds)), dim3(256), 0, st, nitems, ncs, run, d, u, out, ldk, out32);
 }
 
 void launch_residual32(hipStream_t st, int n, int ldk, const int* rowptr, const int* colind, const double* lval,
