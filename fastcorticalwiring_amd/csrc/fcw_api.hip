@@ -497,10 +497,9 @@ struct fcw_ctx {
     // The first FP32 right-hand side, w.X32 = w.X rounded to FP32, is written by the divergence.
     template <typename PT>
     void poisson_refined_with(Workspace& w, const PT* panel) {
-        EventPair* e = nullptr;
         solve32(w, panel, w.Z.p, 1, opt.ir_steps == 0 ? w.Dt.p : nullptr);
         for (int it = 0; it < opt.ir_steps; ++it) {
-            e = ev_begin(w, kResidual, 1);
+            EventPair* e = ev_begin(w, kResidual, 1);
             if (res_grouped)
                 fcw::launch_residual_grouped(w.stream, div_dev(), ldk, rowptr.p, colind.p, res_cloc.p, res_aval.p,
                                              w.X.p, w.Z.p, w.X32.p);
