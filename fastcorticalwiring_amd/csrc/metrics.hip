@@ -7,10 +7,8 @@
 //            For r on the bin grid, sum(dmax < r) <= A(r) <= sum(dmin < r), which
 //            brackets every target radius to a few bins before any exact
 //            evaluation; the counts bound how many faces a bracket can hold.
-//   phase 2  brackets: a band [kL wb, H] per target radius and one per
-//            supplementary sample radius (bracketed from the brackets of the two
-//            radii it lies between); each band gets an exact-size slot range in
-//            the per-source scratch.
+//   phase 2  brackets: a band [kL wb, H] per target radius; each band gets an
+//            exact-size slot range in the per-source scratch.
 //   phase 3  face pass: every face whose [dmin, dmax] meets a band is copied,
 //            pre-sorted (with its area and index), into that band. Phase 1
 //            records the distance range of every 256-face chunk, so chunks that
@@ -20,7 +18,11 @@
 //            face's exact area inside r.
 //   phase 4  safeguarded Newton on the exact piecewise-quadratic A(r) (one wave
 //            per scale), perimeter by the coarea formula at the root
-//   phase 5  exact areas at the supplementary sample radii, one band each
+//   phase 5  exact areas at the supplementary sample radii, now known: one face
+//            pass over the chunks that meet any sample's shell (faces with dmax
+//            bin >= the sample's bin and dmin below its radius), each face adding
+//            its exact area inside each such radius (no band lists: every sample
+//            is evaluated once)
 //   phase 6  outputs and the cleaned (radius, area) samples
 //
 // Face passes are coalesced and issue all loads of a step before any is used;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     // shifted already)
     const double sh = md.shift_src ? md.shift_src[(int64_t)src_pos[j] * md.shift_ld + j] : 0.0;
     const int S = md.S, q = md.q, NE = (S > 0 ? (S - 1) * q : 0);
-    const int NB = S + NE;  // bands: scale s at s (q + 1), its samples (s, jj) at s (q + 1) + 1 + jj
+    const int NB = S;  // one band per scale (the supplementary samples are summed by a face pass in phase 5)
     double* scal = out.scal + (int64_t)j * kSlotCount;
     double* radii = out.radii + (int64_t)j * S;
     double* perim = out.perim + (int64_t)j * S;
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __shared__ double hi_r[kMaxScales], r_out[kMaxScales], p_out[kMaxScales];
     __shared__ float ch_lo[kMaxChunks], ch_hi[kMaxChunks];  // per 256-face chunk: min dmin (rounded
                                                              // down), max dmax (rounded up)
-    __shared__ int s_nbins, s_ovf;
+    __shared__ int s_nbins, s_ovf, s_nsamp;
     __shared__ int s_list[kMaxChunks], s_wcnt[NW], s_nlist;  // face chunks a pass visits, in order
     __shared__ double s_skip_lo, s_skip_hi, s_cut;
     __shared__ double s_wb;
@@ -676,26 +678,12 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     // Band brackets. An empty band has kL = INT_MAX (no dmax bin reaches it) and H = -inf
     // before the first active scale or +inf after the last.
     for (int i = tid; i < NB; i += NT) {
-        const int s = i / (q + 1), jj = i - s * (q + 1) - 1;
+        const int s = i;
         int kL = INT_MAX;
-        double H = (s < S && act[s] == 2) ? INFINITY : -INFINITY;
-        if (jj < 0) {
-            if (act[s] == 1) {
-                kL = lo_k[s];
-                H = hi_r[s];
-            }
-        } else if (act[s] == 1 && act[s + 1] == 1) {
-            // the sample radius is increasing in both radii it lies between; one bin of slack
-            // on each side covers the rounding of exp/log
-            const double fr = md.fracs[jj];
-            const double l0 = (double)lo_k[s] * wb, l1 = fmax((double)lo_k[s + 1] * wb, l0);
-            const double h0 = hi_r[s], h1 = fmax(hi_r[s + 1], h0);
-            const double rlo = (l0 > 0.0) ? log_mix(l0, l1, fr) : 0.0;
-            const double rhi = log_mix(h0, h1, fr);
-            kL = max(0, (int)floor(rlo / wb) - 1);
-            H = rhi + wb;
-        } else if (act[s + 1] == 2) {
-            H = INFINITY;
+        double H = act[s] == 2 ? INFINITY : -INFINITY;
+        if (act[s] == 1) {
+            kL = lo_k[s];
+            H = hi_r[s];
         }
         b_kL[i] = kL;
         b_H[i] = H;
@@ -856,7 +844,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     for (int s = wid; s < S; s += NW) {
         if (act[s] != 1) continue;
         const double T = md.targets[s];
-        const BandRef B = band_ref(s * (q + 1));
+        const BandRef B = band_ref(s);
         const double Ain = Hmax[min(B.kL, nbins)];
         const int klo = lo_k[s];
         const double br_lo = (double)klo * wb, br_hi = hi_r[s];
@@ -932,32 +920,87 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     __syncthreads();
 
     // ---- phase 5: supplementary samples between consecutive radii --------------------------
-    for (int e = tid; e < NE; e += NT) {
-        const int s = e / q, jj = e - s * q;
-        const double r0 = r_out[s], r1 = r_out[s + 1];
-        double rr = NAN;
-        if (r0 == r0 && r1 == r1 && r0 > 0.0 && r1 > r0) rr = log_mix(r0, r1, md.fracs[jj]);
-        U.x.rho[e] = rr;
-        U.x.acc[e] = NAN;
+    // A(r) = (area of faces with dmax bin < bin(r)) + the exact area inside r of every face with
+    // dmax bin >= bin(r) and dmin < r. The sample radii increase with their index, so the valid
+    // ones are listed in order (U.x.key: sample index, U.x.pos: its bin, U.x.pair_r: its radius)
+    // and a face's samples form one run of that list.
+    if (tid == 0) {
+        int nv = 0;
+        for (int e = 0; e < NE; ++e) {
+            const int s = e / q, jj = e - s * q;
+            const double r0 = r_out[s], r1 = r_out[s + 1];
+            double rr = NAN;
+            if (r0 == r0 && r1 == r1 && r0 > 0.0 && r1 > r0) rr = log_mix(r0, r1, md.fracs[jj]);
+            U.x.rho[e] = rr;
+            U.x.acc[e] = rr == rr ? 0.0 : NAN;
+            if (rr == rr) {
+                U.x.key[nv] = e;
+                U.x.pos[nv] = min(bin_of(rr, wb, nbins), (int)(rr / wb));
+                U.x.pair_r[nv] = rr;
+                ++nv;
+            }
+        }
+        s_nsamp = nv;
     }
     __syncthreads();
-    for (int e = wid; e < NE; e += NW) {
-        const double rr = U.x.rho[e];
-        if (rr != rr) continue;
-        const int s = e / q, jj = e - s * q;
-        BandRef B = band_ref(s * (q + 1) + 1 + jj);
-        if (!((double)B.kL * wb <= rr && rr <= B.H)) {
-            // outside its band (cannot happen with the slack above): every face, own bracket
-            B.kL = min(bin_of(rr, wb, nbins), (int)(rr / wb));
-            B.H = rr;
-            B.full = true;
-            if (lane == 0) atomicOr(&s_ovf, 4);
+    const int nsv = s_nsamp;
+    if (nsv > 0) {
+        // first listed sample whose radius exceeds x
+        auto first_above = [&](double x) {
+            int lo = 0, hi = nsv;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (U.x.pair_r[mid] > x) hi = mid; else lo = mid + 1;
+            }
+            return lo;
+        };
+        auto sample_step = [&](const FaceBatch& fb) {
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                double fa = fb.d0[u], fbv = fb.d1[u], fc = fb.d2[u];
+                if (!(fb.f[u] < m && isfinite(fa) && isfinite(fbv) && isfinite(fc))) continue;
+                sort3(fa, fbv, fc);
+                const int kmax = bin_of(fc, wb, nbins);
+                for (int i = first_above(fa); i < nsv && U.x.pos[i] <= kmax; ++i) {
+                    double x, dx;
+                    cdf_sorted(fa, fbv, fc, fb.A[u], U.x.pair_r[i], x, dx);
+                    atomicAdd(&U.x.acc[U.x.key[i]], x);
+                }
+            }
+        };
+        // a chunk can hold a sample's face only if its range reaches that sample's bin (one bin
+        // of slack) and starts below its radius
+        auto meets_samples = [&](int c) {
+            const double clo = ch_in_lds ? (double)ch_lo[c] : (double)gch_lo(c);
+            const double chi = ch_in_lds ? (double)ch_hi[c] : (double)gch_hi(c);
+            const int i = first_above(clo);
+            return i < nsv && (double)(U.x.pos[i] - 1) * wb <= chi;
+        };
+        const int nchunks5 = (m + kWave * UNR - 1) / (kWave * UNR);
+        if (chunked && nchunks5 <= kMaxChunks) {
+            const int nlist = chunk_list(nchunks5, meets_samples, s_list, s_wcnt, s_nlist);
+            FaceBatch cur, nxt;
+            int k = wid;
+            if (k < nlist) load_faces<false>(faces, mesh.larea, d, sh, m, s_list[k] * (kWave * UNR), lane, cur);
+            nxt = cur;
+            for (; k < nlist; k += NW) {
+                if (k + NW < nlist) load_faces<false>(faces, mesh.larea, d, sh, m, s_list[k + NW] * (kWave * UNR), lane, nxt);
+                sample_step(cur);
+                cur = nxt;
+            }
+        } else {
+            for (int base = wid * kWave * UNR; base < m; base += NT * UNR) {
+                if (chunked && !meets_samples(base / (kWave * UNR))) continue;
+                FaceBatch fb;
+                load_faces<false>(faces, mesh.larea, d, sh, m, base, lane, fb);
+                sample_step(fb);
+            }
         }
-        double sA, sdA, sP;
-        band_sums(B, rr, false, pool, pstride, faces, d, sh, mesh.larea, mesh.g11, mesh.g12, mesh.g22, m, wb, nbins, sA,
-                  sdA, sP);
-        const double A = Hmax[min(B.kL, nbins)] + wave_sum(sA);
-        if (lane == 0) U.x.acc[e] = A;
+        __syncthreads();
+        for (int i = tid; i < nsv; i += NT) {
+            const int e = U.x.key[i];
+            U.x.acc[e] += Hmax[min(U.x.pos[i], nbins)];
+        }
     }
     __syncthreads();
 
