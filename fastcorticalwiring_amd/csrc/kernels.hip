@@ -4,6 +4,8 @@
 // n x ldk (one vertex per row, one source per column).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <climits>
 #include <cmath>
 #include <cstdint>
@@ -168,16 +170,15 @@ __device__ inline double rsqrt_refined(double q) {
     return tiny ? y * 0x1p+128 : y;
 }
 
-// One incidence's term -A (grad u . ga) / |grad u| for one column (0 where grad u vanishes:
-// |grad u| > 1e-300, fastcw_lean.py:433, holds for every positive q). Branch-free, so that the
-// terms of several incidences interleave.
-__device__ inline double divergence_term(double u0, double u1, double u2, const double* G) {
+// One incidence's term -A (grad u . ga) / |grad u| for one column, added to the row sum as
+// fma(-A (grad u . ga), 1 / |grad u|, acc) (one rounding per term, the same in every path); no term
+// where grad u vanishes (|grad u| > 1e-300, fastcw_lean.py:433, holds for every positive q).
+__device__ inline double divergence_add(double acc, double u0, double u1, double u2, const double* G) {
     const double du1 = u1 - u0;
     const double du2 = u2 - u0;
     const double num = fma(du2, G[1], du1 * G[0]);
     const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
-    const double r = rsqrt_refined(qq > 0.0 ? qq : 1.0);
-    return qq > 0.0 ? -num * r : 0.0;
+    return qq > 0.0 ? fma(-num, rsqrt_refined(qq), acc) : acc;
 }
 
 // 1/sqrt(q) for q well inside the range where the hardware estimate is accurate (q >= 2^-767):
@@ -213,8 +214,9 @@ __global__ void incidence_geometry_kernel(int ninc, const int* __restrict__ vf, 
 __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, int run, DivDev d, const double* __restrict__ u,
                                                          double* __restrict__ out, int ldk, float* __restrict__ out32) {
     __shared__ double vals[kDivVerts][kDivSlice];  // the group's distinct vertex rows, this slice
-    __shared__ double sgeo[5][kDivInc];
-    __shared__ unsigned sloc[kDivInc];
+    __shared__ double sgeo[5 * kDivInc];  // per incidence its five coefficients, as in d.geo
+    // per incidence: the LDS byte offsets of its three vertex rows in vals, (o0 | o1 << 16, o2)
+    __shared__ uint2 soff[kDivInc];
     __shared__ int rptr[kDivGroupRows + 1];
     __shared__ int sexp[4][kDivSlice];   // per wave and column: largest exponent of its staged values
 
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             const double u0 = u[(int64_t)fv[0] * ldk + col];
             const double u1 = u[(int64_t)fv[1] * ldk + col];
             const double u2 = u[(int64_t)fv[2] * ldk + col];
-            acc += divergence_term(u0, u1, u2, d.geo + 5 * (int64_t)e);
+            acc = divergence_add(acc, u0, u1, u2, d.geo + 5 * (int64_t)e);
         }
         out[(int64_t)r0 * ldk + col] = acc;
         if (out32) out32[(int64_t)r0 * ldk + col] = (float)acc;
@@ -278,45 +280,65 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     {
         const int ni = d.vf_ptr[r0 + nr] - e0;
         for (int i = tid; i < ni; i += blockDim.x) {
-            sloc[i] = d.loc[e0 + i];
+            const unsigned L = d.loc[e0 + i];
+            constexpr unsigned kRow = kDivSlice * sizeof(double);
+            soff[i] = make_uint2((L & 63u) * kRow | (((L >> 6) & 63u) * kRow) << 16, ((L >> 12) & 63u) * kRow);
             const double* gp = d.geo + 5 * (int64_t)(e0 + i);
 #pragma unroll
-            for (int c = 0; c < 5; ++c) sgeo[c][i] = gp[c];
+            for (int c = 0; c < 5; ++c) sgeo[5 * i + c] = gp[c];
         }
     }
     __syncthreads();
+    // One incidence of this lane's column, read from LDS: -A (grad u . ga) and 1 / |grad u|. The
+    // fast path assumes |grad u|^2 >= 2^-767 (the scaled values keep it far above that) and only
+    // records whether an incidence breaks that; a row with such an incidence in any lane is summed
+    // again the general way (no term where grad u vanishes, rescaled estimate where it is tiny),
+    // which adds every other term with the same bits.
+    const char* const vrow = reinterpret_cast<const char*>(&vals[0][0]) + lane * sizeof(double);
+    auto val = [&](unsigned off) { return *reinterpret_cast<const double*>(vrow + off); };
+    // C incidences from e: every LDS read first, then the arithmetic; acc += each term in order
+    auto step = [&](auto width, int e, double acc, uint64_t& odd) -> double {
+        constexpr int C = decltype(width)::value;
+        uint2 P[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) P[q] = soff[e + q];
+        double u0[C], u1[C], u2[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            u0[q] = val(P[q].x & 0xffffu);
+            u1[q] = val(P[q].x >> 16);
+            u2[q] = val(P[q].y);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // every value read is issued before any is used
+        double nn[C], rr[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const double* G = sgeo + 5 * (e + q);
+            const double du1 = u1[q] - u0[q], du2 = u2[q] - u0[q];
+            nn[q] = -fma(du2, G[1], du1 * G[0]);
+            const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
+            odd |= __ballot(!(qq >= 0x1p-767));
+            rr[q] = rsqrt_newton(qq);
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q) acc = fma(nn[q], rr[q], acc);
+        return acc;
+    };
     for (int r = w; r < nr; r += 4) {
         const int rs = rptr[r], re = rptr[r + 1];
         double acc = 0.0;
-        bool tiny = false;
-        // kDivC incidences per step, all computed (the padding repeats the last one), then added
-        // in CSR order with the padding masked
-        for (int e = rs; e < re; e += kDivC) {
-            double t[kDivC];
-#pragma unroll
-            for (int q = 0; q < kDivC; ++q) {
-                const int ec = min(e + q, re - 1);
-                const unsigned L = sloc[ec];
-                const double u0 = vals[L & 63u][lane];
-                const double du1 = vals[(L >> 6) & 63u][lane] - u0;
-                const double du2 = vals[(L >> 12) & 63u][lane] - u0;
-                const double num = fma(du2, sgeo[1][ec], du1 * sgeo[0][ec]);
-                const double qq = fma(du1, fma(du2, sgeo[3][ec], du1 * sgeo[2][ec]), du2 * (du2 * sgeo[4][ec]));
-                tiny |= qq > 0.0 && qq < 0x1p-767;
-                // 0 where q <= 0 (the estimate is then NaN or infinite) by a bit mask, so that the
-                // compiler keeps the six terms branch-free
-                const uint64_t keep = qq > 0.0 ? ~0ull : 0ull;
-                t[q] = __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, -num * rsqrt_newton(qq)) & keep);
-            }
-#pragma unroll
-            for (int q = 0; q < kDivC; ++q) acc += (e + q < re) ? t[q] : 0.0;
-        }
-        if (__any(tiny)) {
+        uint64_t odd = 0;  // lanes with an incidence off the fast path
+        // kDivC incidences per step, then the rest one by one (in CSR order throughout)
+        int e = rs;
+        for (; e + kDivC <= re; e += kDivC) acc = step(std::integral_constant<int, kDivC>{}, e, acc, odd);
+        for (; e < re; ++e) acc = step(std::integral_constant<int, 1>{}, e, acc, odd);
+        if (odd) {
             acc = 0.0;
-            for (int e = rs; e < re; ++e) {
-                const unsigned L = sloc[e];
-                const double G[5] = {sgeo[0][e], sgeo[1][e], sgeo[2][e], sgeo[3][e], sgeo[4][e]};
-                acc += divergence_term(vals[L & 63u][lane], vals[(L >> 6) & 63u][lane], vals[(L >> 12) & 63u][lane], G);
+            for (int ec = rs; ec < re; ++ec) {
+                const uint2 P = soff[ec];
+                acc = divergence_add(acc, *reinterpret_cast<const double*>(vrow + (P.x & 0xffffu)),
+                                     *reinterpret_cast<const double*>(vrow + (P.x >> 16)),
+                                     *reinterpret_cast<const double*>(vrow + P.y), sgeo + 5 * ec);
             }
         }
         out[(int64_t)(r0 + r) * ldk + col] = acc;
