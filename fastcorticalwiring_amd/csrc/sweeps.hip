@@ -173,11 +173,11 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     // split-BF16 stages, depth contiguous: A hi / lo (tile row, depth), B hi / lo (column, depth)
     __shared__ __align__(16) unsigned short bfa[BF ? 2 : 1][BF ? TM : 1][BF ? kBfPad : 1];
     __shared__ __align__(16) unsigned short bfb[BF ? 2 : 1][BF ? TN : 1][BF ? kBfPad : 1];
-    // Stage per-row indices: backward -> boundary positions (b rows), forward -> two child
-    // contributor rows for each of the node's s own rows.
+    // Stage per-row indices: backward -> the source row of every depth index (own rows as positions
+    // in Y, boundary rows as positions in Z), forward -> two child contributor rows for each of the
+    // node's s own rows.
     if (IDX_LDS && SPLIT != 2) {
         if (MODE == Sweep::Backward) {
-            // source row of every depth index: own rows as positions, boundary rows as -(pos + 1).
             // The boundary-map loads are issued together from clamped indices (only the uniform
             // per-workgroup bounds are branched on), then written.
             constexpr int R = 2 * kIdxCap / kThreads;
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
                 for (int u = 0; u < R; ++u) {
                     const int i = tid + u * kThreads;
-                    if (i < f) sidx[i] = i < s ? start + i : -(q[u] + 1);
+                    if (i < f) sidx[i] = i < s ? start + i : q[u];
                 }
             } else {
                 for (int i = tid; i < f; i += kThreads) sidx[i] = start + i;
@@ -199,15 +199,29 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
     }
     __syncthreads();
 
-    // Every tile load below is unconditional: out-of-range elements read a clamped in-bounds
-    // address. fetch() only issues the loads of the next step (and records which elements are
-    // kept, subtracted or negated); stash() applies that after the current step's MFMAs, so no
-    // arithmetic on a loaded value forces a wait before the matrix cores have their work.
+    // Tile loads. Panel (A) element e of a thread: tile row / depth from e; right-hand-side (B)
+    // element i: column c (the same for all of a thread's elements, since 256 is a multiple of TN)
+    // and depth row l = k0 + wrow + (256 / TN) i, where wrow = tid / TN is the same in every lane of
+    // a wave (TN >= 64): each B element is one row of the right-hand side, whose address is
+    // wave-uniform (computed on the scalar unit), plus this lane's column.
+    //
+    // Every load is unconditional and in bounds (indices clamped). Depth steps past the node's
+    // depth are made zero on the panel side only: a panel value at depth k >= K is staged as 0, so
+    // the right-hand-side values it meets (a clamped, finite row) add nothing. The backward sweep's
+    // boundary rows enter negated ([Y_S ; -Z_B]); the sign is put on the panel values at depths
+    // s..f-1 instead of on the right-hand side (the products are the same). fetch() only issues the
+    // loads of the next step; stash() applies masks and signs after the current step's MFMAs.
+    const int wrow = __builtin_amdgcn_readfirstlane(tid / TN);
+    const int c = n0 + (tid & (TN - 1));
     PT ra[LA];
     T rb[LB], ru[KIDS && MODE == Sweep::Forward ? LB : 1], rv[KIDS && MODE == Sweep::Forward ? LB : 1];
-    unsigned keep_a = 0, keep_b = 0, bit_u = 0;
+    int kA = 0;  // backward: this thread's panel depth (the same for its LA elements) of the staged step
+    unsigned fwd_keep = 0;  // forward: panel elements inside the node's columns
+    auto row_ptr = [&](const T* base, int row) {  // wave-uniform row base
+        return base + (int64_t)__builtin_amdgcn_readfirstlane(row) * ldk;
+    };
     auto fetch = [&](int k0) {
-        keep_a = keep_b = bit_u = 0;
+        fwd_keep = 0;
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int e = tid + i * kThreads;
@@ -219,62 +233,57 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 prow = k0 + e % kTileK;  // P^T: walk a panel column
                 pcol = t.m0 + e / kTileK;
             }
-            keep_a |= (prow < f && pcol < s) ? (1u << i) : 0u;
+            if (MODE == Sweep::Forward) fwd_keep |= pcol < s ? (1u << i) : 0u;
             ra[i] = P[min(prow, f - 1) + (int64_t)min(pcol, s - 1) * f];
         }
+        if (MODE == Sweep::Backward) kA = k0 + (tid & (kTileK - 1));
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
-            const int e = tid + i * kThreads;
-            const int c = n0 + e % TN, l = k0 + e / TN;
+            const int l = k0 + wrow + (kThreads / TN) * i;  // wave-uniform
             if (MODE == Sweep::Forward) {
-                // F rows: own RHS row minus up to two children's update rows
+                // F rows: own RHS row minus up to two children's update rows (a missing contributor
+                // reads the zero row)
                 const int lc = min(l, s - 1);
-                const int pos = start + lc;
-                keep_b |= (l < s) ? (1u << i) : 0u;
-                rb[i] = Bsrc[(int64_t)pos * ldk + c];
+                rb[i] = row_ptr(Bsrc, start + lc)[c];
                 if constexpr (KIDS) {
-                    // a missing contributor reads the zero row
-                    const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * pos];
-                    const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * pos + 1];
-                    ru[i] = U[(int64_t)(ca >= 0 ? ca : zrow) * ldk + c];
-                    rv[i] = U[(int64_t)(cb >= 0 ? cb : zrow) * ldk + c];
+                    const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * (start + lc)];
+                    const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * (start + lc) + 1];
+                    ru[i] = row_ptr(U, ca >= 0 ? ca : zrow)[c];
+                    rv[i] = row_ptr(U, cb >= 0 ? cb : zrow)[c];
                 }
             } else {
-                // [Y_S ; -Z_B] rows: own rows from Y, boundary rows negated from Z
+                // [Y_S ; Z_B] rows: own rows from Y, boundary rows from Z
                 const int lc = min(l, f - 1);
-                int code;  // position p of an own row, -(p + 1) of a boundary row
-                if (IDX_LDS)
-                    code = sidx[lc];
-                else
-                    code = lc < s ? start + lc : -(B[lc - s] + 1);
-                const bool bnd = code < 0;
-                const int prow = bnd ? -code - 1 : code;
-                keep_b |= (l < f) ? (1u << i) : 0u;
-                bit_u |= bnd ? (1u << i) : 0u;
-                // one load from either buffer, its base chosen by integer select (a pointer select
-                // is split by the compiler into two loads under branches)
-                const uintptr_t base = bnd ? reinterpret_cast<uintptr_t>(Zsrc) : reinterpret_cast<uintptr_t>(Bsrc);
-                rb[i] = reinterpret_cast<const T*>(base)[(int64_t)prow * ldk + c];
+                const int prow = IDX_LDS ? sidx[lc] : (lc < s ? start + lc : B[lc - s]);
+                rb[i] = row_ptr(lc < s ? Bsrc : Zsrc, prow)[c];
             }
         }
     };
     // value of staged right-hand-side element i
     auto bval = [&](int i) -> T {
-        T v;
         if constexpr (MODE == Sweep::Forward && KIDS)
-            v = rb[i] - ru[i] - rv[i];
-        else if constexpr (MODE == Sweep::Forward)
-            v = rb[i];
+            return rb[i] - ru[i] - rv[i];
         else
-            v = ((bit_u >> i) & 1u) ? -rb[i] : rb[i];
-        return v;
+            return rb[i];
+    };
+    // panel element i as staged: 0 outside the node's depth, negated at the backward's boundary depths
+    auto aval = [&](int i) -> PT {
+        if constexpr (MODE == Sweep::Forward) {
+            return ((fwd_keep >> i) & 1u) ? ra[i] : PT(0);
+        } else if constexpr (BF) {
+            const unsigned v = kA < s ? ra[i] : ra[i] ^ 0x80008000u;  // both bf16 halves negated
+            return kA < f ? v : 0u;
+        } else {
+            const T v = kA < s ? ra[i] : -ra[i];
+            return kA < f ? v : T(0);
+        }
     };
     auto stash = [&]() {
         if constexpr (BF) {
 #pragma unroll
             for (int i = 0; i < LA; ++i) {
                 const int e = tid + i * kThreads;
-                const unsigned v = ((keep_a >> i) & 1u) ? ra[i] : 0u;
+                const unsigned v = aval(i);
                 const int m = (MODE == Sweep::Forward) ? e % TM : e / kTileK;
                 const int k = (MODE == Sweep::Forward) ? e / TM : e % kTileK;
                 bfa[0][m][k] = (unsigned short)(v & 0xffffu);
@@ -283,7 +292,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
             for (int i = 0; i < LB; ++i) {
                 const int e = tid + i * kThreads;
-                const unsigned sv = ((keep_b >> i) & 1u) ? split_bf16(bval(i)) : 0u;
+                const unsigned sv = split_bf16(bval(i));
                 bfb[0][e % TN][e / TN] = (unsigned short)(sv & 0xffffu);
                 bfb[1][e % TN][e / TN] = (unsigned short)(sv >> 16);
             }
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
         for (int i = 0; i < LA; ++i) {
             const int e = tid + i * kThreads;
-            const T v = ((keep_a >> i) & 1u) ? ra[i] : T(0);
+            const T v = aval(i);
             if (MODE == Sweep::Forward)
                 As[(e / TM) * kLdA + e % TM] = v;
             else
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
 #pragma unroll
         for (int i = 0; i < LB; ++i) {
             const int e = tid + i * kThreads;
-            Bs[(e / TN) * kLdB + e % TN] = ((keep_b >> i) & 1u) ? bval(i) : T(0);
+            Bs[(e / TN) * kLdB + e % TN] = bval(i);
         }
     };
 
