@@ -374,7 +374,8 @@ __global__ __launch_bounds__(256) void residual32_kernel(int n, int ldk, const i
                 xv[q] = x[(int64_t)cq * ldk + col];
             }
 #pragma unroll
-            for (int q = 0; q < kC; ++q) ax += (e + q < e1) ? a[q] * xv[q] : 0.0;
+            for (int q = 0; q < kC; ++q)
+                if (e + q < e1) ax = fma(a[q], xv[q], ax);
         }
         v -= ax;
     }
@@ -383,10 +384,12 @@ __global__ __launch_bounds__(256) void residual32_kernel(int n, int ldk, const i
 
 // The same residual over the divergence's row groups: a workgroup stages the values of x at the
 // group's distinct vertices for one 64-column slice in LDS (every row's matrix neighbours are
-// vertices of its faces, so they are among them), then each wave takes rows of the group, one lane
-// per column, reading x only from LDS. aval = the CSR values of L + shift M (shift on the
-// diagonal); cloc = each CSR entry's local vertex index in its row's group. Sums in CSR order, as
-// residual32_kernel.
+// vertices of its faces, so they are among them), with the group's row pointers, matrix values and
+// local column indices (when the group has at most kResEntries entries), then each wave takes rows
+// of the group, one lane per column, reading only LDS. aval = the CSR values of L + shift M (shift
+// on the diagonal); cloc = each CSR entry's local vertex index in its row's group. Each row is
+// summed in CSR order (fused multiply-adds), so the result does not depend on scheduling.
+constexpr int kResEntries = 512;
 __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int ncs, int run, DivDev d,
                                                                const int* __restrict__ rowptr,
                                                                const int* __restrict__ colind,
@@ -396,6 +399,9 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
                                                                const double* __restrict__ x, float* __restrict__ r,
                                                                int ldk) {
     __shared__ double vals[kDivVerts][kDivSlice];
+    __shared__ double sa[kResEntries];
+    __shared__ unsigned char sc[kResEntries];
+    __shared__ int rptr[kDivGroupRows + 1];
     const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
     if (item >= nitems) return;
     const int g = item / ncs, cs = item - g * ncs;
@@ -408,10 +414,18 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
         // a row beyond the group limits, alone: x straight from global memory
         if (w != 0) return;
         double ax = 0.0;
-        for (int e = rowptr[r0]; e < rowptr[r0 + 1]; ++e) ax += aval[e] * x[(int64_t)colind[e] * ldk + col];
+        for (int e = rowptr[r0]; e < rowptr[r0 + 1]; ++e) ax = fma(aval[e], x[(int64_t)colind[e] * ldk + col], ax);
         r[(int64_t)r0 * ldk + col] = (float)(b[(int64_t)r0 * ldk + col] - ax);
         return;
     }
+    const int eg = rowptr[r0], ne = rowptr[r0 + nr] - eg;
+    const bool staged = ne <= kResEntries;  // the same for the whole workgroup
+    if (tid <= nr) rptr[tid] = rowptr[r0 + tid] - eg;
+    if (staged)
+        for (int i = tid; i < ne; i += blockDim.x) {
+            sa[i] = aval[eg + i];
+            sc[i] = cloc[eg + i];
+        }
     if (nv > 0) {
         constexpr int Q = kDivVerts / 4;
         double xv[Q];
@@ -425,8 +439,8 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
     __syncthreads();
     for (int rr = w; rr < nr; rr += 4) {
         const int p = r0 + rr;
-        const int e0 = rowptr[p], e1 = rowptr[p + 1];
         const double bv = b[(int64_t)p * ldk + col];
+        const int e0 = rptr[rr], e1 = rptr[rr + 1];
         double ax = 0.0;
         constexpr int kC = 8;
         for (int e = e0; e < e1; e += kC) {
@@ -435,13 +449,14 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
 #pragma unroll
             for (int q = 0; q < kC; ++q) {
                 const int ec = min(e + q, e1 - 1);
-                a[q] = aval[ec];
-                c[q] = cloc[ec];
+                a[q] = staged ? sa[ec] : aval[eg + ec];
+                c[q] = staged ? sc[ec] : cloc[eg + ec];
             }
 #pragma unroll
             for (int q = 0; q < kC; ++q) xq[q] = vals[c[q]][lane];
 #pragma unroll
-            for (int q = 0; q < kC; ++q) ax += (e + q < e1) ? a[q] * xq[q] : 0.0;
+            for (int q = 0; q < kC; ++q)
+                if (e + q < e1) ax = fma(a[q], xq[q], ax);
         }
         r[(int64_t)p * ldk + col] = (float)(bv - ax);
     }

@@ -246,8 +246,10 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(
                 const int lc = min(l, s - 1);
                 rb[i] = row_ptr(Bsrc, start + lc)[c];
                 if constexpr (KIDS) {
-                    const int ca = IDX_LDS ? sidx[2 * lc] : cown[2 * (start + lc)];
-                    const int cb = IDX_LDS ? sidx[2 * lc + 1] : cown[2 * (start + lc) + 1];
+                    // contributor rows taken as uniform values first, so the choice of the zero row
+                    // for a missing one is scalar work too
+                    const int ca = __builtin_amdgcn_readfirstlane(IDX_LDS ? sidx[2 * lc] : cown[2 * (start + lc)]);
+                    const int cb = __builtin_amdgcn_readfirstlane(IDX_LDS ? sidx[2 * lc + 1] : cown[2 * (start + lc) + 1]);
                     ru[i] = row_ptr(U, ca >= 0 ? ca : zrow)[c];
                     rv[i] = row_ptr(U, cb >= 0 ? cb : zrow)[c];
                 }
