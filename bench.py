@@ -4,9 +4,10 @@
 Metric: cortical sources per second on the 164k-vertex folded surface (icosphere
 L7, R = 70 mm, SH folding to degree 32, 0.3h tangential jitter, 10% masked cap:
 148,165 cortical vertices, 295,882 faces), 12 area scales 0.002-0.05 and 3
-samples between scales. One step = one RHS block of `--block` sources through
-heat solve -> fused divergence -> Poisson solve -> metric kernel, with outputs
-copied back. The mesh, operators and factors are resident in HBM before the
+samples between scales. One step = one batch of `--streams` RHS blocks of
+`--block` sources (the blocks the device keeps in flight, one per HIP stream),
+each through heat solve -> fused divergence -> Poisson solve -> metric kernel,
+with outputs copied back. The mesh, operators and factors are resident in HBM before the
 timed region; the default run times every cortical source once, in each
 precision form (plain FP64; FP32 Poisson sweeps refined against the FP64
 residual; split-BF16x3 Poisson sweeps refined the same way).
@@ -193,7 +194,7 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, chec
                     health_h=prep["health_h"], bin_width=prep["width"])
     setup = time.perf_counter() - t0
     order = prep["order"]
-    k = ctx.block
+    k = ctx.block * streams  # sources per step: one block per stream
     if steps is None:
         src = order
         steps = (order.size + k - 1) // k
@@ -218,15 +219,15 @@ def run_gpu(prep, precision, block, steps, warmup, profile_blocks, streams, chec
     # distance fields handed to the caller's host memory (fcw_distance_block: solve, shift, one
     # device-to-host copy of k x n doubles per block), the adapter's path; PCIe-inclusive, reported
     # beside the resident-input rate, never as it
-    nblk = 4
-    ctx.distance_block(order[:k])
+    nblk, kb = 4, ctx.block
+    ctx.distance_block(order[:kb])
     t1 = time.perf_counter()
     for b in range(nblk):
-        ctx.distance_block(np.resize(order, (b + 2) * k)[(b + 1) * k:])
-    out["distances_to_host_sources_per_s"] = nblk * k / (time.perf_counter() - t1)
+        ctx.distance_block(np.resize(order, (b + 2) * kb)[(b + 1) * kb:])
+    out["distances_to_host_sources_per_s"] = nblk * kb / (time.perf_counter() - t1)
     # per-launch timing with HIP events on the context's streams
     ctx.set_profiling(True)
-    ctx.wiring(np.resize(order, profile_blocks * k))
+    ctx.wiring(np.resize(order, profile_blocks * kb))
     out["kernel_ms"] = ctx.kernel_times()
     out["kernel_log"] = ctx.kernel_log()
     out["stage_s"] = ctx.stage_times()
@@ -349,7 +350,8 @@ def load_traffic(kernel):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="RHS blocks timed (default: every cortical source once)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="batches of --streams RHS blocks timed (default: every cortical source once)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", type=int, default=3)
     ap.add_argument("--block", type=int, default=256, help="RHS block width when not autotuned")
@@ -495,6 +497,7 @@ def main():
             "surface_vertices": int(prep["n_full"]),
             "faces": int(prep["F"].shape[0]),
             "block": int(best["info"]["block"]),
+            "sources_per_step": int(best["info"]["block"]) * int(args.streams),
             "leaf_size": int(leaf),
             "tile_rows": int(tile) if tile else "chosen per tree level",
             "streams": int(args.streams),
