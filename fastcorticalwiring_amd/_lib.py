@@ -126,7 +126,7 @@ EXPORTED_SYMBOLS = (
     "fcw_set_metrics", "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times",
     "fcw_set_profiling", "fcw_kernel_times", "fcw_build_info", "fcw_last_error", "fcw_host_factor_solve",
     "fcw_calibration_copy", "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse",
-    "fcw_wiring_sources_noalloc", "fcw_kernel_log", "fcw_host_intrinsic_cover",
+    "fcw_wiring_sources_noalloc", "fcw_kernel_log", "fcw_host_intrinsic_cover", "fcw_divergence_block",
 )
 
 _lib = None
@@ -158,6 +158,7 @@ def lib():
     L.fcw_destroy.argtypes = [_vp]
     L.fcw_get_info.argtypes = [_vp, ctypes.POINTER(Info)]
     L.fcw_distance_block.argtypes = [_vp, _ip, ctypes.c_int, _dp]
+    L.fcw_divergence_block.argtypes = [_vp, _dp, ctypes.c_int, _dp]
     L.fcw_set_metrics.argtypes = [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double, ctypes.c_int]
@@ -183,7 +184,7 @@ def lib():
                                    _dp, ctypes.POINTER(ctypes.c_uint64)]
     L.fcw_mesh_prep.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, ctypes.POINTER(ctypes.c_ubyte), ctypes.c_int,
                                 ctypes.c_int, ctypes.POINTER(MeshPrepOut)]
-    for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_set_metrics",
+    for name in ("fcw_create", "fcw_destroy", "fcw_get_info", "fcw_distance_block", "fcw_divergence_block", "fcw_set_metrics",
                  "fcw_wiring_sources", "fcw_geometry_fields", "fcw_stage_times", "fcw_set_profiling",
                  "fcw_kernel_times", "fcw_build_info", "fcw_host_factor_solve", "fcw_calibration_copy",
                  "fcw_host_intrinsic", "fcw_mesh_prep", "fcw_host_analyse", "fcw_wiring_sources_noalloc"):

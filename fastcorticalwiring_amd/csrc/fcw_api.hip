@@ -1391,6 +1391,27 @@ int fcw_distance_block(fcw_ctx* C, const int* src, int k, double* out) {
     });
 }
 
+int fcw_divergence_block(fcw_ctx* C, const double* u, int k, double* out) {
+    return guard([&] {
+        if (!C || !u || !out) throw Fail("null argument");
+        if (k <= 0 || k > C->ldk) throw Fail("column count must be in 1..block (" + std::to_string(C->ldk) + ")");
+        const int n = C->n, ldk = C->ldk;
+        const std::vector<int>& iperm = C->sym.iperm;
+        // vertex v's row at its position, columns past k zero
+        std::vector<double> h((size_t)n * ldk, 0.0);
+        for (int v = 0; v < n; ++v)
+            std::memcpy(h.data() + (size_t)iperm[v] * ldk, u + (size_t)v * k, sizeof(double) * k);
+        Workspace& w = *C->ws[0];
+        HIPCHK(hipMemcpyAsync(w.Z.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, w.stream));
+        fcw::launch_divergence(w.stream, C->div_dev(), ldk, w.Z.p, w.X.p, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h.data(), w.X.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, w.stream));
+        HIPCHK(hipStreamSynchronize(w.stream));
+        for (int v = 0; v < n; ++v)
+            std::memcpy(out + (size_t)v * k, h.data() + (size_t)iperm[v] * ldk, sizeof(double) * k);
+    });
+}
+
 int fcw_set_metrics(fcw_ctx* C, const double* targets, int S, const double* fracs, int q, const int* boundary, int nb,
                     const double* vertex_w, double eps, double health_h, double bin_width, double cap_fraction,
                     double rtol, int max_iter) {

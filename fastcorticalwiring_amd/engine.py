@@ -95,6 +95,18 @@ class HeatContext:
         _lib.check(_lib.lib().fcw_distance_block(self._h, _lib.iptr(src), src.size, _lib.dptr(out)))
         return out
 
+    def divergence(self, u):
+        """Fused gradient -> normalise -> divergence of the (n, k) fields u (k <= block), as
+        fused_normalized_divergence (fastcw_lean.py:416-442): (n, k) float64."""
+        U = np.ascontiguousarray(u, dtype=np.float64)
+        if U.ndim == 1:
+            U = U[:, None]
+        if U.shape[0] != self.n or not 1 <= U.shape[1] <= self.block:
+            raise ValueError(f"fields must be (n, k) with n = {self.n} and 1 <= k <= {self.block}")
+        out = np.empty_like(U)
+        _lib.check(_lib.lib().fcw_divergence_block(self._h, _lib.dptr(U), U.shape[1], _lib.dptr(out)))
+        return out
+
     def distances(self, sources):
         src = np.asarray(sources, dtype=np.int64).reshape(-1)
         parts = [self.distance_block(src[i:i + self.block]) for i in range(0, src.size, self.block)]

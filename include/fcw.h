@@ -124,6 +124,13 @@ int fcw_get_info(const fcw_ctx* ctx, fcw_info* out);
  * ConsistentBatchHeatEngine.compute_distance_batch (fastcw_lean.py:504-531). k <= block. */
 int fcw_distance_block(fcw_ctx* ctx, const int* src, int k, double* out);
 
+/* The fused gradient -> normalise -> divergence of k fields on the context's operator faces:
+ * out[v][j] = sum over faces f at v of A_f (X_f . grad B_v), X_f = -grad u_j / |grad u_j| on f (no term
+ * where |grad u_j| vanishes). u and out are n x k, row per vertex (input vertex order), as the (n, k)
+ * arrays of fused_normalized_divergence (fastcw_lean.py:416-442), which this replaces; the same kernel
+ * runs between the heat and Poisson solves of fcw_distance_block and fcw_wiring_sources. k <= block. */
+int fcw_divergence_block(fcw_ctx* ctx, const double* u, int k, double* out);
+
 /* Per-source metric configuration (core_analysis.py:1513-1514, 1536, 1588, 705):
  * targets = sorted scale * total area (S <= 32), fracs = extra-sample fractions (q), boundary =
  * submesh boundary vertices, vertex_w = vertex areas (n), eps = MSD threshold (1e-6), health_h =
