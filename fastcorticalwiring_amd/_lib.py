@@ -53,6 +53,7 @@ class Options(ctypes.Structure):
         ("band_cap", ctypes.c_int),
         ("graphs", ctypes.c_int),
         ("tile_rows", ctypes.c_int),
+        ("index_staging", ctypes.c_int),
     ]
 
 

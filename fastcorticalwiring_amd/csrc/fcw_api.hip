@@ -1025,6 +1025,7 @@ void fcw_default_options(fcw_options* opt) {
     opt->split_depth = 0;
     opt->band_cap = 6144;
     opt->graphs = 1;
+    opt->index_staging = 1;
 }
 
 const char* fcw_last_error(void) { return g_err.c_str(); }
@@ -1319,8 +1320,8 @@ int fcw_create(const double* Vin, int n, const int* F, int m, const fcw_options*
         C->bwd_tiles.upload(bt.data(), std::max<size_t>(1, bt.size()), tb);
         for (int l = 0; l < S.nlevels; ++l) C->info.split_levels += (C->fwd_split[l] > 1) + (C->bwd_split[l] > 1);
         const int max_s = S.nnodes ? *std::max_element(S.node_s.begin(), S.node_s.end()) : 0;
-        C->fwd_idx_lds = max_s <= fcw::kSweepIdxCap;
-        C->bwd_idx_lds = S.max_front <= 2 * fcw::kSweepIdxCap;
+        C->fwd_idx_lds = opt.index_staging != 0 && max_s <= fcw::kSweepIdxCap;
+        C->bwd_idx_lds = opt.index_staging != 0 && S.max_front <= 2 * fcw::kSweepIdxCap;
         double t4 = now_s();
         fcw_info& I = C->info;
         I.n = n;

@@ -26,7 +26,7 @@ class HeatContext:
 
     def __init__(self, vertices, faces, t_coef=1.0, t=None, poisson_shift=1e-10, block=256, leaf_size=64,
                  precision="fp64", ir_steps=None, device=0, host_threads=0, streams=8, sparse_rhs=True, robust=False,
-                 split_depth=0, band_cap=0, graphs=True, tile_rows=0):
+                 split_depth=0, band_cap=0, graphs=True, tile_rows=0, index_staging=True):
         V = np.ascontiguousarray(vertices, dtype=np.float64)
         F = np.ascontiguousarray(faces, dtype=np.int32)
         if V.ndim != 2 or V.shape[1] != 3:
@@ -53,6 +53,7 @@ class HeatContext:
             opt.band_cap = int(band_cap)
         opt.graphs = 1 if graphs else 0
         opt.tile_rows = int(tile_rows)
+        opt.index_staging = 1 if index_staging else 0
         handle = ctypes.c_void_p()
         _lib.check(L.fcw_create(_lib.dptr(V), V.shape[0], _lib.iptr(F), F.shape[0], ctypes.byref(opt),
                                 ctypes.byref(handle)))

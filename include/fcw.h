@@ -77,6 +77,9 @@ typedef struct {
                               direction among 64, 32 and 16 by a model of the work each executes (32 needs a
                               block width that is a multiple of 128; 16, backward only, a multiple of 256);
                               16, 32 or 64: the smallest allowed height >= this on every level */
+    int index_staging;     /* 1 (default): each sweep workgroup stages its node's row maps (boundary rows,
+                              child contributor rows) in LDS when every node's fit (fronts up to 2,048 rows);
+                              0: every sweep reads them from global memory, the path larger fronts take */
 } fcw_options;
 
 typedef struct {

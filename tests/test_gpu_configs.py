@@ -173,6 +173,28 @@ def test_every_row_tile_height_matches_the_oracle(tile_rows, precision):
                   w["sample_a"], fx, tol, f"tile_rows={tile_rows} {precision}")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_ir", "bf16x3_ir"])
+def test_row_maps_from_global_memory_give_the_same_results(precision):
+    """The sweeps read each node's row maps (boundary rows, child contributor rows) from global
+    memory when a front is too large for LDS; forced on every node, that path gives the distances of
+    the LDS-staged one bit for bit (the same rows feed the same products), and the oracle's
+    metrics."""
+    from fastcorticalwiring_amd.engine import HeatContext
+
+    fx, mesh, _ = folded(1)
+    tol = TOL[precision]
+    with HeatContext(mesh.V, mesh.F, block=256, precision=precision, index_staging=False) as ctx, \
+            HeatContext(mesh.V, mesh.F, block=256, precision=precision) as staged:
+        D = ctx.distances(fx["src"])
+        assert np.array_equal(D, staged.distances(fx["src"]))
+        err, rel = check_distances(D, fx, tol)
+        assert err < tol["dist"] and rel < tol["dist"], (err, rel)
+        set_metrics(ctx, fx, mesh)
+        w = ctx.wiring(fx["src"])
+    check_outputs(w["scalars"], w["radii"], w["perimeters"], w["extra_r"], w["extra_a"], w["sample_r"],
+                  w["sample_a"], fx, tol, f"row maps from global memory, {precision}")
+
+
 def test_tile_rows_option_is_validated():
     from fastcorticalwiring_amd._lib import FcwError
     from fastcorticalwiring_amd.engine import HeatContext
