@@ -254,6 +254,9 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
     // mesh spacings, over which u changes by a few decades at most, so |grad u|^2 stays far above
     // the range where the hardware estimate needs the rescaling of rsqrt_refined. A wave that meets
     // such a value anyway (a vanishing gradient) sums the row again the general way.
+    // the wave index as a uniform value: vertex indices, row bounds and incidence offsets below are
+    // then scalar values and the loops over them scalar branches
+    const int wu = __builtin_amdgcn_readfirstlane(w);
     if (nv > 0) {
         // wave w loads vertices w, w + 4, ...: all its loads first (in-range repeats of the last
         // vertex pad the count), then the LDS stores
@@ -261,21 +264,25 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
         double x[Q];
         const int* gv = d.verts + v0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) x[q] = u[(int64_t)gv[min(w + 4 * q, nv - 1)] * ldk + col];
-        int emax = INT_MIN;
-#pragma unroll
         for (int q = 0; q < Q; ++q) {
-            int ex = 0;
-            (void)frexp(x[q], &ex);
-            emax = (w + 4 * q < nv && x[q] != 0.0) ? max(emax, ex) : emax;
+            const int v = __builtin_amdgcn_readfirstlane(gv[min(wu + 4 * q, nv - 1)]);
+            x[q] = u[(int64_t)v * ldk + col];
         }
-        sexp[w][lane] = emax;
+        // largest exponent of the lane's values: that of the largest high word of their magnitudes
+        // (the padding repeats a staged vertex, so it changes nothing); none when every high word
+        // is zero (values of zero, or below 2^-1054)
+        unsigned hmax = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) hmax = max(hmax, (unsigned)__double2hiint(x[q]) & 0x7fffffffu);
+        int emax = INT_MIN;
+        if (hmax != 0) (void)frexp(__hiloint2double((int)hmax, 0), &emax);
+        sexp[wu][lane] = emax;
         __syncthreads();
         emax = max(max(sexp[0][lane], sexp[1][lane]), max(sexp[2][lane], sexp[3][lane]));
         const double sc = ldexp(1.0, emax == INT_MIN ? 0 : -emax);
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (w + 4 * q < nv) vals[w + 4 * q][lane] = x[q] * sc;
+            if (wu + 4 * q < nv) vals[wu + 4 * q][lane] = x[q] * sc;
     }
     {
         const int ni = d.vf_ptr[r0 + nr] - e0;
@@ -324,8 +331,8 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
         for (int q = 0; q < C; ++q) acc = fma(nn[q], rr[q], acc);
         return acc;
     };
-    for (int r = w; r < nr; r += 4) {
-        const int rs = rptr[r], re = rptr[r + 1];
+    for (int r = wu; r < nr; r += 4) {
+        const int rs = __builtin_amdgcn_readfirstlane(rptr[r]), re = __builtin_amdgcn_readfirstlane(rptr[r + 1]);
         double acc = 0.0;
         uint64_t odd = 0;  // lanes with an incidence off the fast path
         // kDivC incidences per step, then the rest one by one (in CSR order throughout)

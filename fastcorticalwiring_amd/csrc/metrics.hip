@@ -147,7 +147,7 @@ __device__ inline void load_faces(const int* __restrict__ faces, const double* _
 // takes part); returns their number.
 template <typename Keep>
 __device__ int chunk_list(int nchunks, Keep keep, int* list, int* wcnt, int& nlist) {
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = __builtin_amdgcn_readfirstlane(tid / kWave);
     if (tid == 0) nlist = 0;
     __syncthreads();
     for (int c0 = 0; c0 < nchunks; c0 += NT) {
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
                                                       const int* __restrict__ src_pos, MetricsOut out,
                                                       int health_mode) {
     const int j = blockIdx.x;
-    const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+    const int tid = threadIdx.x, lane = lane_id(), wid = __builtin_amdgcn_readfirstlane(tid / kWave);
     const int n = mesh.n, m = mesh.m;
     const int* __restrict__ faces = mesh.faces;
     const double* __restrict__ d = Dt + (int64_t)j * n;
