@@ -148,12 +148,14 @@ __global__ __launch_bounds__(256) void clear_rows_kernel(const int* __restrict__
 // kDivVerts distinct vertices (at most kDivInc incidences, kDivGroupRows rows); with the
 // nested-dissection order a group of about 20 rows of a degree-6 mesh touches about 57 vertices.
 // A workgroup takes one (group, 64-column slice): it loads each distinct vertex's 64 values once
-// into LDS (one 512-byte run per vertex, all loads in flight together) with the group's
-// coefficients and each incidence's three local vertex indices, then each wave sums rows of the
-// group, one lane per column, reading everything from LDS. A vertex row is read from L2 about 2.7
-// times per slice instead of 18 times (three per incidence). Each row sums its incidences in CSR
-// order, so the result does not depend on scheduling. A row that alone exceeds the limits (a
-// vertex of degree above 63) forms a group of its own that reads its values from global memory.
+// into LDS (one 512-byte run per vertex, all loads in flight together) with each incidence's three
+// local vertex indices, then each wave sums rows of the group, one lane per column, reading the
+// values from LDS and each incidence's five coefficients (the same for every lane) by scalar loads
+// into scalar registers: as LDS broadcasts they took about two thirds of the LDS cycles. A vertex
+// row is read from L2 about 2.7 times per slice instead of 18 times (three per incidence). Each
+// row sums its incidences in CSR order, so the result does not depend on scheduling. A row that
+// alone exceeds the limits (a vertex of degree above 63) forms a group of its own that reads its
+// values from global memory.
 //
 // Workgroups are handed to the 8 XCDs round robin; the grid maps them so that each XCD walks its
 // own contiguous run of (group, slice) items, the slices of a group one after another, and the
@@ -173,7 +175,8 @@ __device__ inline double rsqrt_refined(double q) {
 // One incidence's term -A (grad u . ga) / |grad u| for one column, added to the row sum as
 // fma(-A (grad u . ga), 1 / |grad u|, acc) (one rounding per term, the same in every path); no term
 // where grad u vanishes (|grad u| > 1e-300, fastcw_lean.py:433, holds for every positive q).
-__device__ inline double divergence_add(double acc, double u0, double u1, double u2, const double* G) {
+template <typename GP>  // pointer to the five coefficients (generic or constant address space)
+__device__ inline double divergence_add(double acc, double u0, double u1, double u2, GP G) {
     const double du1 = u1 - u0;
     const double du2 = u2 - u0;
     const double num = fma(du2, G[1], du1 * G[0]);
@@ -214,7 +217,6 @@ __global__ void incidence_geometry_kernel(int ninc, const int* __restrict__ vf, 
 __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, int run, DivDev d, const double* __restrict__ u,
                                                          double* __restrict__ out, int ldk, float* __restrict__ out32) {
     __shared__ double vals[kDivVerts][kDivSlice];  // the group's distinct vertex rows, this slice
-    __shared__ double sgeo[5 * kDivInc];  // per incidence its five coefficients, as in d.geo
     // per incidence: the LDS byte offsets of its three vertex rows in vals, (o0 | o1 << 16, o2)
     __shared__ uint2 soff[kDivInc];
     __shared__ int rptr[kDivGroupRows + 1];
@@ -290,22 +292,30 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
             const unsigned L = d.loc[e0 + i];
             constexpr unsigned kRow = kDivSlice * sizeof(double);
             soff[i] = make_uint2((L & 63u) * kRow | (((L >> 6) & 63u) * kRow) << 16, ((L >> 12) & 63u) * kRow);
-            const double* gp = d.geo + 5 * (int64_t)(e0 + i);
-#pragma unroll
-            for (int c = 0; c < 5; ++c) sgeo[5 * i + c] = gp[c];
         }
     }
     __syncthreads();
-    // One incidence of this lane's column, read from LDS: -A (grad u . ga) and 1 / |grad u|. The
+    // One incidence of this lane's column: -A (grad u . ga) and 1 / |grad u|. The
     // fast path assumes |grad u|^2 >= 2^-767 (the scaled values keep it far above that) and only
     // records whether an incidence breaks that; a row with such an incidence in any lane is summed
     // again the general way (no term where grad u vanishes, rescaled estimate where it is tiny),
     // which adds every other term with the same bits.
     const char* const vrow = reinterpret_cast<const char*>(&vals[0][0]) + lane * sizeof(double);
     auto val = [&](unsigned off) { return *reinterpret_cast<const double*>(vrow + off); };
-    // C incidences from e: every LDS read first, then the arithmetic; acc += each term in order
+    // The coefficients are not written while the kernel runs, so they are read through the
+    // constant address space, whose wave-uniform loads are scalar loads.
+    typedef const __attribute__((address_space(4))) double cf64;
+    cf64* const ggeo = (cf64*)(d.geo + 5 * (int64_t)e0);
+    // C incidences from e: the step's coefficients (scalar loads) and offsets first, then every
+    // value read, then the arithmetic; acc += each term in order
     auto step = [&](auto width, int e, double acc, uint64_t& odd) -> double {
         constexpr int C = decltype(width)::value;
+        double gc[C][5];
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+#pragma unroll
+            for (int c = 0; c < 5; ++c) gc[q][c] = ggeo[5 * (e + q) + c];
+        __builtin_amdgcn_sched_barrier(0);
         uint2 P[C];
 #pragma unroll
         for (int q = 0; q < C; ++q) P[q] = soff[e + q];
@@ -320,7 +330,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
         double nn[C], rr[C];
 #pragma unroll
         for (int q = 0; q < C; ++q) {
-            const double* G = sgeo + 5 * (e + q);
+            const double* G = gc[q];
             const double du1 = u1[q] - u0[q], du2 = u2[q] - u0[q];
             nn[q] = -fma(du2, G[1], du1 * G[0]);
             const double qq = fma(du1, fma(du2, G[3], du1 * G[2]), du2 * (du2 * G[4]));
@@ -345,7 +355,7 @@ __global__ __launch_bounds__(256) void divergence_kernel(int nitems, int ncs, in
                 const uint2 P = soff[ec];
                 acc = divergence_add(acc, *reinterpret_cast<const double*>(vrow + (P.x & 0xffffu)),
                                      *reinterpret_cast<const double*>(vrow + (P.x >> 16)),
-                                     *reinterpret_cast<const double*>(vrow + P.y), sgeo + 5 * ec);
+                                     *reinterpret_cast<const double*>(vrow + P.y), ggeo + 5 * ec);
             }
         }
         out[(int64_t)(r0 + r) * ldk + col] = acc;
