@@ -401,11 +401,12 @@ __global__ __launch_bounds__(256) void residual32_kernel(int n, int ldk, const i
 
 // The same residual over the divergence's row groups: a workgroup stages the values of x at the
 // group's distinct vertices for one 64-column slice in LDS (every row's matrix neighbours are
-// vertices of its faces, so they are among them), with the group's row pointers, matrix values and
-// local column indices (when the group has at most kResEntries entries), then each wave takes rows
-// of the group, one lane per column, reading only LDS. aval = the CSR values of L + shift M (shift
-// on the diagonal); cloc = each CSR entry's local vertex index in its row's group. Each row is
-// summed in CSR order (fused multiply-adds), so the result does not depend on scheduling.
+// vertices of its faces, so they are among them), with the group's row pointers and local column
+// indices (when the group has at most kResEntries entries), then each wave takes rows of the
+// group, one lane per column, reading x from LDS and the matrix values (the same for every lane)
+// by scalar loads. aval = the CSR values of L + shift M (shift on the diagonal); cloc = each CSR
+// entry's local vertex index in its row's group. Each row is summed in CSR order (fused
+// multiply-adds), so the result does not depend on scheduling.
 constexpr int kResEntries = 512;
 __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int ncs, int run, DivDev d,
                                                                const int* __restrict__ rowptr,
@@ -416,7 +417,6 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
                                                                const double* __restrict__ x, float* __restrict__ r,
                                                                int ldk) {
     __shared__ double vals[kDivVerts][kDivSlice];
-    __shared__ double sa[kResEntries];
     __shared__ unsigned char sc[kResEntries];
     __shared__ int rptr[kDivGroupRows + 1];
     const int item = (blockIdx.x % kXcds) * run + blockIdx.x / kXcds;
@@ -439,44 +439,61 @@ __global__ __launch_bounds__(256) void residual_grouped_kernel(int nitems, int n
     const bool staged = ne <= kResEntries;  // the same for the whole workgroup
     if (tid <= nr) rptr[tid] = rowptr[r0 + tid] - eg;
     if (staged)
-        for (int i = tid; i < ne; i += blockDim.x) {
-            sa[i] = aval[eg + i];
-            sc[i] = cloc[eg + i];
-        }
+        for (int i = tid; i < ne; i += blockDim.x) sc[i] = cloc[eg + i];
+    const int wu = __builtin_amdgcn_readfirstlane(w);  // vertex indices and row bounds as scalars
     if (nv > 0) {
         constexpr int Q = kDivVerts / 4;
         double xv[Q];
         const int* gv = d.verts + v0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) xv[q] = x[(int64_t)gv[min(w + 4 * q, nv - 1)] * ldk + col];
+        for (int q = 0; q < Q; ++q) {
+            const int v = __builtin_amdgcn_readfirstlane(gv[min(wu + 4 * q, nv - 1)]);
+            xv[q] = x[(int64_t)v * ldk + col];
+        }
 #pragma unroll
         for (int q = 0; q < Q; ++q)
-            if (w + 4 * q < nv) vals[w + 4 * q][lane] = xv[q];
+            if (wu + 4 * q < nv) vals[wu + 4 * q][lane] = xv[q];
     }
     __syncthreads();
-    for (int rr = w; rr < nr; rr += 4) {
-        const int p = r0 + rr;
-        const double bv = b[(int64_t)p * ldk + col];
-        const int e0 = rptr[rr], e1 = rptr[rr + 1];
-        double ax = 0.0;
-        constexpr int kC = 8;
-        for (int e = e0; e < e1; e += kC) {
-            double a[kC], xq[kC];
-            unsigned c[kC];
+    // The matrix values (the same for every lane, not written while the kernel runs) are read
+    // through the constant address space: scalar loads into scalar registers, once per step.
+    typedef const __attribute__((address_space(4))) double cf64;
+    cf64* const ga = (cf64*)(aval + eg);
+    // Steps of kC entries; past the row's end a step repeats its last entry with the value 0, whose
+    // product adds nothing (exactly: the values are finite and the sum starts at +0).
+    auto rows = [&](auto local) {
+        constexpr bool kStaged = decltype(local)::value;
+        for (int rr = wu; rr < nr; rr += 4) {
+            const int p = r0 + rr;
+            const double bv = b[(int64_t)p * ldk + col];
+            const int e0 = __builtin_amdgcn_readfirstlane(rptr[rr]), e1 = __builtin_amdgcn_readfirstlane(rptr[rr + 1]);
+            double ax = 0.0;
+            constexpr int kC = 8;
+            for (int e = e0; e < e1; e += kC) {
+                double a[kC], xq[kC];
+                unsigned c[kC];
 #pragma unroll
-            for (int q = 0; q < kC; ++q) {
-                const int ec = min(e + q, e1 - 1);
-                a[q] = staged ? sa[ec] : aval[eg + ec];
-                c[q] = staged ? sc[ec] : cloc[eg + ec];
+                for (int q = 0; q < kC; ++q) a[q] = ga[min(e + q, e1 - 1)];
+                __builtin_amdgcn_sched_barrier(0);  // the step's loads issued together
+#pragma unroll
+                for (int q = 0; q < kC; ++q) a[q] = e + q < e1 ? a[q] : 0.0;
+#pragma unroll
+                for (int q = 0; q < kC; ++q) {
+                    const int ec = min(e + q, e1 - 1);
+                    c[q] = kStaged ? sc[ec] : cloc[eg + ec];
+                }
+#pragma unroll
+                for (int q = 0; q < kC; ++q) xq[q] = vals[c[q]][lane];
+#pragma unroll
+                for (int q = 0; q < kC; ++q) ax = fma(a[q], xq[q], ax);
             }
-#pragma unroll
-            for (int q = 0; q < kC; ++q) xq[q] = vals[c[q]][lane];
-#pragma unroll
-            for (int q = 0; q < kC; ++q)
-                if (e + q < e1) ax = fma(a[q], xq[q], ax);
+            r[(int64_t)p * ldk + col] = (float)(bv - ax);
         }
-        r[(int64_t)p * ldk + col] = (float)(bv - ax);
-    }
+    };
+    if (staged)
+        rows(std::true_type{});
+    else
+        rows(std::false_type{});
 }
 
 // aval[e] = lval[e] plus shift * mass[p] on row p's diagonal entry (the residual's matrix values).
