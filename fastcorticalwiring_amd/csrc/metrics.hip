@@ -296,9 +296,12 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
     }
 }
 
-__global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md, const double* __restrict__ Dt,
-                                                      const int* __restrict__ src_pos, MetricsOut out,
-                                                      int health_mode) {
+// Four waves per SIMD, so that two workgroups (two sources) share a CU: the kernel wants 137
+// registers per lane, which leaves room for one workgroup only (three waves per SIMD); held to 128
+// it spills 22 of them to scratch, and a 512-source launch takes 0.39 ms instead of 0.54.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void metrics_kernel(
+    MeshDev mesh, MetricsDev md, const double* __restrict__ Dt, const int* __restrict__ src_pos, MetricsOut out,
+    int health_mode) {
     const int j = blockIdx.x;
     const int tid = threadIdx.x, lane = lane_id(), wid = __builtin_amdgcn_readfirstlane(tid / kWave);
     const int n = mesh.n, m = mesh.m;
@@ -841,8 +844,8 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
     };
 
     // ---- phase 4: safeguarded Newton per scale (one wave per scale) ------------------------
-    for (int s = wid; s < S; s += NW) {
-        if (act[s] != 1) continue;
+    auto newton = [&](int s) {
+        if (act[s] != 1) return;
         const double T = md.targets[s];
         const BandRef B = band_ref(s);
         const double Ain = Hmax[min(B.kL, nbins)];
@@ -916,6 +919,23 @@ __global__ __launch_bounds__(NT) void metrics_kernel(MeshDev mesh, MetricsDev md
             p_out[s] = Pres;
             ev_out[s] = n_eval;
         }
+    };
+    // Scales by wave. The targets increase with the scale index and so do the bands, the cost of
+    // each evaluation: with NW < S <= 2 NW the 2 NW - S largest scales run alone and the others in
+    // pairs, smallest with largest (12 scales on 8 waves: about the largest scale's time instead of
+    // the 4th and 12th together); otherwise round robin. Each scale's result does not depend on the
+    // wave that computes it.
+    if (S > NW && S <= 2 * NW) {
+        const int alone = 2 * NW - S;
+        if (wid < alone) {
+            newton(S - 1 - wid);
+        } else {
+            const int p = wid - alone, rest = S - alone;
+            newton(p);
+            newton(rest - 1 - p);
+        }
+    } else {
+        for (int s = wid; s < S; s += NW) newton(s);
     }
     __syncthreads();
 
