@@ -219,15 +219,17 @@ struct BandRef {
     bool full;
 };
 
-__device__ void band_sums(const BandRef& B, double r, bool want_p, const double* __restrict__ pool, int64_t pstride,
-                          const int* __restrict__ faces, const double* __restrict__ d, double sh,
-                          const double* __restrict__ larea, const double* __restrict__ g11,
-                          const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
-                          int nbins, double& sA, double& sdA, double& sP) {
+// U band entries per lane in flight per step; each lane adds its entries in increasing index order
+// whatever U is, so the sums do not depend on it.
+template <int U>
+__device__ void band_sums_u(const BandRef& B, double r, bool want_p, const double* __restrict__ pool, int64_t pstride,
+                            const int* __restrict__ faces, const double* __restrict__ d, double sh,
+                            const double* __restrict__ larea, const double* __restrict__ g11,
+                            const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
+                            int nbins, double& sA, double& sdA, double& sP) {
     const int lane = lane_id();
     sA = sdA = sP = 0.0;
     if (!B.full) {
-        constexpr int U = 4;
         const double* pa = pool + B.off;
         const double* pb = pa + pstride;
         const double* pc = pb + pstride;
@@ -296,9 +298,22 @@ __device__ void band_sums(const BandRef& B, double r, bool want_p, const double*
     }
 }
 
-// Four waves per SIMD, so that two workgroups (two sources) share a CU: the kernel wants 137
-// registers per lane, which leaves room for one workgroup only (three waves per SIMD); held to 128
-// it spills 22 of them to scratch, and a 512-source launch takes 0.39 ms instead of 0.54.
+// The perimeter evaluations (once or twice per scale) hold three more values per face in flight,
+// so they take two entries per lane per step instead of four: with four the kernel needs 137
+// registers per lane, one workgroup per CU; with two, 124, and two sources share a CU.
+__device__ void band_sums(const BandRef& B, double r, bool want_p, const double* __restrict__ pool, int64_t pstride,
+                          const int* __restrict__ faces, const double* __restrict__ d, double sh,
+                          const double* __restrict__ larea, const double* __restrict__ g11,
+                          const double* __restrict__ g12, const double* __restrict__ g22, int m, double wb,
+                          int nbins, double& sA, double& sdA, double& sP) {
+    if (want_p)
+        band_sums_u<2>(B, r, true, pool, pstride, faces, d, sh, larea, g11, g12, g22, m, wb, nbins, sA, sdA, sP);
+    else
+        band_sums_u<4>(B, r, false, pool, pstride, faces, d, sh, larea, g11, g12, g22, m, wb, nbins, sA, sdA, sP);
+}
+
+// Four waves per SIMD, so that two workgroups (two sources) share a CU (the registers allow it
+// without spilling: see band_sums); a 512-source launch takes 0.39 ms instead of 0.54 with one.
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void metrics_kernel(
     MeshDev mesh, MetricsDev md, const double* __restrict__ Dt, const int* __restrict__ src_pos, MetricsOut out,
     int health_mode) {
