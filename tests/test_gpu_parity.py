@@ -182,6 +182,35 @@ def test_geometry_fields_euclid_and_non_lipschitz(c2):
     assert np.all(out["scalars"][:, 8] == 0)  # no band overflow
 
 
+@pytest.mark.parametrize("nscales", [1, 5, 8, 9, 13, 16, 17, 24])
+def test_geometry_fields_any_scale_count(c2, nscales):
+    """The metric kernel's per-scale Newton runs one wave per scale, assigned by the number of
+    scales (at most one per wave, the largest alone and the rest paired, or round robin): every
+    assignment against the oracle's per_source_geometry."""
+    from fastcorticalwiring_amd.engine import HeatContext
+    from oracle import geometry as og
+
+    s, mesh, _, H = c2
+    total = float(np.sum(mesh.vertex_areas))
+    T = np.geomspace(0.001, 0.05, nscales) * total
+    fr = np.array([0.25, 0.5, 0.75])
+    rng = np.random.default_rng(5)
+    src = rng.choice(mesh.n, 8, replace=False)
+    D = np.linalg.norm(mesh.V[None, :, :] - mesh.V[src][:, None, :], axis=2)
+    with HeatContext(mesh.V, mesh.F, block=64) as ctx:
+        ctx.set_metrics(T, fr, mesh.boundary, mesh.vertex_areas, eps=1e-6, health_h=mesh.health_h,
+                        bin_width=mesh.width)
+        out = ctx.geometry_fields(D, src)
+    ref = og.per_source_geometry(D, mesh.F, mesh.lean_area, mesh.g11, mesh.g12, mesh.g22, mesh.width, T, fr)
+    for key, rk in (("radii", "radii"), ("perimeters", "perim"), ("extra_r", "extra_r"), ("extra_a", "extra_a")):
+        a, b = out[key], ref[rk]
+        assert a.shape == b.shape, key
+        assert np.array_equal(np.isnan(a), np.isnan(b)), key
+        fin = np.isfinite(b)
+        e = np.max(rel_err(a[fin], b[fin])) if fin.any() else 0.0
+        assert e < 1e-9, (key, e)
+
+
 def test_wiring_metrics_match_oracle_loop(c2):
     from oracle import analysis as oa
 
